@@ -1,0 +1,125 @@
+"""Headline benchmark: MC move attempts/s of the checkerboard sweep at
+N = 32000, rho = 0.85 (fcc 4*20^3, dr_max = 0.1), on one MI355X.
+
+A "step" is one sweep (N attempted moves).  Inputs are resident in HBM when
+the timed region starts.  Prints one JSON line (see DESIGN.md for fields).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per attempted move of the sweep: per sweep N*(24+24)
+# FP64 positions read+written, N*(4+4) cell-slot ids, C*(4+4) cell counts;
+# at N = 32000 on a 32^3 grid 2,054,144 B / 32000 = 64.19 B per attempt
+N_PART = 32000
+GRID_CELLS = 32 ** 3
+BYTES_PER_SWEEP = N_PART * 48 + N_PART * 8 + GRID_CELLS * 8
+BYTES_PER_ATTEMPT = BYTES_PER_SWEEP / N_PART
+
+
+def cpu_baseline(target_s: float = 12.0) -> dict:
+    """The oracle's serial reference-algorithm sweep (1 core) on the same
+    system: reference grid neigh_list 1.05 (31^3 cells), max_part 10."""
+    from oracle import oracle as O
+    o = O.System(2, 20, rho=0.85, neigh_dr=1.05, max_part=10, dr_max=0.1, seed=1)
+    w, _ = o.time_sweeps(10)
+    n = max(10, int(target_s / max(w / 10, 1e-6)))
+    w, c = o.time_sweeps(n)
+    rate = N_PART * n / w
+    return {"value": rate, "unit": "MC move attempts/s", "cores": 1, "kind": "port",
+            "sample": f"{n} serial sweeps ({n * N_PART} part_move attempts) of the same "
+                      f"N=32000 rho=0.85 system from the fcc lattice, oracle/liborc.so, "
+                      f"gcc -O2, {w:.1f} s wall"}
+
+
+def traffic_from_profile():
+    path = os.path.join(ROOT, "profiles", "sweep_phase_n32000_pmc.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return float(json.load(f)["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(independent replicas shard across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from hsmc_amd import gpu
+    conf, box = gpu.lattice(2, 20, 0.85)
+    ctx = gpu.Context([dict(n=len(conf), box=box, dr_max=0.1, seed=1)])
+    ctx.upload(conf)
+    print(f"[bench] grid {ctx.grid()}  N={len(conf)}", file=sys.stderr, flush=True)
+    tuning = ctx.get_tuning() if args.no_autotune else ctx.autotune(64)
+    print(f"[bench] tuning {tuning}", file=sys.stderr, flush=True)
+
+    ctx.sweeps(args.warmup)
+    ctx.sync()
+    ctx.reset_counters()
+    t0 = time.perf_counter()
+    ms = ctx.time_sweeps(args.steps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    c = ctx.counters()
+    assert c["attempts"] == args.steps * N_PART, c
+    secs = ms / 1e3
+    value = N_PART * args.steps / secs
+    print(f"[bench] {args.steps} sweeps in {secs:.3f} s (events), {wall:.3f} s (wall); "
+          f"acceptance {c['accepted'] / c['attempts']:.4f}", file=sys.stderr, flush=True)
+
+    # dominant kernel: the sublattice-phase kernel, timed per launch
+    avg_ms, launches = ctx.time_phase_kernel(200)
+    per_launch_bytes = BYTES_PER_ATTEMPT * N_PART / 8.0
+    achieved = per_launch_bytes / (avg_ms / 1e3) / 1e9
+    if ctx.any_overlap_scaled([1.0])[0] != 1:
+        raise SystemExit("overlap found after the benchmark sweeps")
+
+    line = {
+        "metric": "MC move attempts/s (N=32k, rho=0.85)",
+        "value": value,
+        "unit": "MC move attempts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (fcc 4*20^3 lattice start, Philox proposals)",
+        "config": {"workload": "NVT hard spheres N=32000 (fcc 4*20^3) rho=0.85 dr_max=0.1, "
+                               "checkerboard sweep on a 32^3 grid",
+                   "n_part": N_PART, "rho": 0.85, "dr_max": 0.1, "grid": list(ctx.grid()),
+                   "tuning": tuning, "acceptance": c["accepted"] / c["attempts"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
+                     "kernel": "k_phase (sublattice phase)", "avg_launch_ms": avg_ms,
+                     "launches_timed": launches,
+                     "algorithmic_bytes_per_launch": per_launch_bytes},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,108 @@
+// ctx.h -- internal layout of an hsmc_gpu_ctx (not part of the C ABI).
+//
+// Device memory of a context (R replicas, P = sum N_r particles, C = sum of
+// sweep-grid cells, A = C/8 active cells per sublattice phase):
+//   x, y, z          FP64[P]     positions by (replica offset + particle id)
+//   prep             int32[P]    replica of each particle
+//   cnt              int32[2*C]  occupancy of each sweep cell (two buffers,
+//                                alternating by sweep parity)
+//   spid             int32[C*cap] particle id in each cell slot
+//   sx, sy, sz       FP64[C*cap] position copy in each cell slot
+//   reps             RepDev[R]   box, grid and RNG key of each replica
+//   aoff             int32[R+1]  prefix of active cells per phase
+//   sweep            uint64      sweeps done (read by every sweep kernel)
+//   ctr              uint64[R*3*16] attempts / accepted / left-cell, spread
+//                                over 16 slots to limit atomic contention
+//   err              uint32      sticky error bits set by kernels
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "hsmc_gpu.h"
+
+namespace hsmc {
+
+constexpr int kSpread = 16;          // counter slots per replica and kind
+constexpr unsigned kErrOverflow = 1u;  // a cell received more than cap occupants
+constexpr unsigned kErrListFull = 2u;  // an LDS candidate list overflowed
+
+struct RepDev {
+  double lx, ly, lz;     // box
+  double hx, hy, hz;     // half box (lx/2.0 ...)
+  double sx, sy, sz;     // sweep-grid cell side (lx/gx ...)
+  double dr_max;         // displacement width
+  int N, gx, gy, gz;     // particles, sweep grid
+  int poff, coff, aoff;  // offsets into particle, cell and active-cell arrays
+  uint32_t k0, k1;       // Philox key
+};
+
+struct RepHost {
+  hsmc_gpu_replica p;
+  int gx, gy, gz;
+  long poff, coff, aoff;
+};
+
+}  // namespace hsmc
+
+struct hsmc_gpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int R = 0;
+  std::vector<hsmc::RepHost> reps;
+  long Ptot = 0, Ctot = 0, Atot = 0;
+  int cap = 8;
+  size_t cell_alloc = 0;  // cells allocated (>= Ctot)
+  hsmc::RepDev *d_reps = nullptr;
+  int *d_aoff = nullptr;
+  int *d_prep = nullptr;
+  double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr;
+  int *d_cnt = nullptr;
+  int *d_spid = nullptr;
+  double *d_sx = nullptr, *d_sy = nullptr, *d_sz = nullptr;
+  unsigned long long *d_sweep = nullptr;
+  unsigned long long *d_ctr = nullptr;
+  unsigned int *d_err = nullptr;
+  hsmc_gpu_tuning tune{32, 8, 32, 0};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_sweeps = 0;
+  hsmc_gpu_tuning graph_tune{0, 0, 0, 0};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string last_error;
+  // configuration version (bumped by anything that moves particles) and the
+  // cached observable grid built for (replica, cutoff, version)
+  unsigned long long version = 1;
+  int aux_replica = -1;
+  double aux_cut = 0.0;
+  unsigned long long aux_version = 0;
+  int aux_n[3] = {0, 0, 0};
+  // scratch buffers of the observables, grown on demand
+  static constexpr int kScratch = 24;
+  void *scratch[kScratch] = {};
+  size_t scratch_bytes[kScratch] = {};
+};
+
+namespace hsmc {
+
+// Record a HIP failure and return HSMC_ERR_HIP.
+int hip_fail(hsmc_gpu_ctx *ctx, hipError_t e, const char *what);
+// Device buffer `slot` of at least `bytes` bytes.
+void *scratch(hsmc_gpu_ctx *ctx, int slot, size_t bytes);
+// (Re)build replica offsets, the sweep grids, and the device cell arrays.
+int layout(hsmc_gpu_ctx *ctx);
+// Sweep-grid cells per axis for a box edge: largest even g with L/g > 1.
+int sweep_grid_cells(double L);
+// Read and clear the kernels' sticky error bits; translate to a status.
+int check_device_errors(hsmc_gpu_ctx *ctx);
+// Invalidate any captured sweep graph.
+void drop_graph(hsmc_gpu_ctx *ctx);
+
+}  // namespace hsmc
+
+#define HSMC_HIP(ctx, call)                                   \
+  do {                                                        \
+    hipError_t e_ = (call);                                   \
+    if (e_ != hipSuccess) return hsmc::hip_fail((ctx), e_, #call); \
+  } while (0)

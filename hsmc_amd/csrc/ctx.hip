@@ -1,0 +1,348 @@
+// ctx.hip -- life cycle, layout and configuration transfer of a context.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "ctx.h"
+#include "device_common.h"
+
+using namespace hsmc;
+
+namespace hsmc {
+
+int hip_fail(hsmc_gpu_ctx *ctx, hipError_t e, const char *what) {
+  if (ctx) {
+    ctx->last_error = std::string(what) + ": " + hipGetErrorString(e);
+  }
+  return HSMC_ERR_HIP;
+}
+
+void *scratch(hsmc_gpu_ctx *ctx, int slot, size_t bytes) {
+  if (slot < 0 || slot >= hsmc_gpu_ctx::kScratch) return nullptr;
+  if (bytes == 0) bytes = 16;
+  if (ctx->scratch_bytes[slot] >= bytes) return ctx->scratch[slot];
+  if (ctx->scratch[slot]) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->scratch[slot]);
+    ctx->scratch[slot] = nullptr;
+    ctx->scratch_bytes[slot] = 0;
+  }
+  size_t want = bytes + bytes / 4;
+  if (hipMalloc(&ctx->scratch[slot], want) != hipSuccess) {
+    ctx->last_error = "scratch allocation failed";
+    return nullptr;
+  }
+  ctx->scratch_bytes[slot] = want;
+  return ctx->scratch[slot];
+}
+
+int sweep_grid_cells(double L) {
+  const double minside = 1.0 + 1e-9;
+  int g = (int)floor(L / minside);
+  if (g & 1) g -= 1;
+  while (g >= 2 && !(L / g > minside)) g -= 2;
+  return g;
+}
+
+void drop_graph(hsmc_gpu_ctx *ctx) {
+  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  ctx->gexec = nullptr;
+  ctx->graph = nullptr;
+  ctx->graph_sweeps = 0;
+}
+
+int layout(hsmc_gpu_ctx *ctx) {
+  long coff = 0, aoff = 0;
+  bool big = false;
+  std::vector<RepDev> rd(ctx->R);
+  std::vector<int> ao(ctx->R + 1);
+  for (int r = 0; r < ctx->R; ++r) {
+    RepHost &h = ctx->reps[r];
+    const hsmc_gpu_replica &p = h.p;
+    h.gx = sweep_grid_cells(p.lx);
+    h.gy = sweep_grid_cells(p.ly);
+    h.gz = sweep_grid_cells(p.lz);
+    if (h.gx < 2 || h.gy < 2 || h.gz < 2) {
+      ctx->last_error = "box edge must exceed 2 diameters for the checkerboard sweep";
+      return HSMC_ERR_NOT_SUPPORTED;
+    }
+    h.coff = coff;
+    h.aoff = aoff;
+    const long cells = (long)h.gx * h.gy * h.gz;
+    coff += cells;
+    aoff += cells / 8;
+    RepDev &d = rd[r];
+    d.lx = p.lx; d.ly = p.ly; d.lz = p.lz;
+    d.hx = p.lx / 2.0; d.hy = p.ly / 2.0; d.hz = p.lz / 2.0;
+    d.sx = p.lx / h.gx; d.sy = p.ly / h.gy; d.sz = p.lz / h.gz;
+    // a cube of side < 2/sqrt(3) holds at most 8 points 1 apart
+    if (d.sx >= 1.1547 || d.sy >= 1.1547 || d.sz >= 1.1547) big = true;
+    d.dr_max = p.dr_max;
+    d.N = p.n_part;
+    d.gx = h.gx; d.gy = h.gy; d.gz = h.gz;
+    d.poff = (int)h.poff; d.coff = (int)h.coff; d.aoff = (int)h.aoff;
+    d.k0 = (uint32_t)p.seed;
+    d.k1 = (uint32_t)(p.seed >> 32);
+    ao[r] = (int)h.aoff;
+  }
+  ao[ctx->R] = (int)aoff;
+  if (coff > (1L << 30)) return HSMC_ERR_NOT_SUPPORTED;
+  ctx->Ctot = coff;
+  ctx->Atot = aoff;
+  const int cap = big ? 16 : 8;
+  const size_t need = (size_t)coff;
+  if (need > ctx->cell_alloc || cap != ctx->cap || !ctx->d_cnt) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->d_cnt); (void)hipFree(ctx->d_spid);
+    (void)hipFree(ctx->d_sx); (void)hipFree(ctx->d_sy); (void)hipFree(ctx->d_sz);
+    ctx->d_cnt = nullptr; ctx->d_spid = nullptr;
+    ctx->d_sx = ctx->d_sy = ctx->d_sz = nullptr;
+    size_t alloc = need + need / 8 + 64;
+    HSMC_HIP(ctx, hipMalloc(&ctx->d_cnt, sizeof(int) * 2 * alloc));
+    HSMC_HIP(ctx, hipMalloc(&ctx->d_spid, sizeof(int) * alloc * cap));
+    HSMC_HIP(ctx, hipMalloc(&ctx->d_sx, sizeof(double) * alloc * cap));
+    HSMC_HIP(ctx, hipMalloc(&ctx->d_sy, sizeof(double) * alloc * cap));
+    HSMC_HIP(ctx, hipMalloc(&ctx->d_sz, sizeof(double) * alloc * cap));
+    ctx->cell_alloc = alloc;
+    ctx->cap = cap;
+  }
+  // keep the phase kernel's LDS staging (27*cap spheres per active cell)
+  // within 64 KiB per workgroup
+  {
+    const size_t lmax = 27 * (size_t)ctx->cap;
+    const size_t per_cell = 3 * lmax * sizeof(double) + (2 * lmax + ctx->cap) * sizeof(int);
+    while (ctx->tune.cells_per_block > 1 &&
+           per_cell * (size_t)ctx->tune.cells_per_block > 64 * 1024)
+      ctx->tune.cells_per_block >>= 1;
+  }
+  HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cnt, 0, sizeof(int) * 2 * ctx->cell_alloc, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_reps, rd.data(), sizeof(RepDev) * ctx->R,
+                               hipMemcpyHostToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_aoff, ao.data(), sizeof(int) * (ctx->R + 1),
+                               hipMemcpyHostToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  drop_graph(ctx);
+  ctx->version++;
+  return HSMC_OK;
+}
+
+int check_device_errors(hsmc_gpu_ctx *ctx) {
+  unsigned int e = 0;
+  HSMC_HIP(ctx, hipMemcpyAsync(&e, ctx->d_err, sizeof(e), hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (e) {
+    unsigned int zero = 0;
+    HSMC_HIP(ctx, hipMemcpy(ctx->d_err, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    ctx->last_error = (e & kErrOverflow) ? "a cell received more occupants than it holds"
+                                         : "an overlap candidate list overflowed";
+    return HSMC_ERR_CELL_OVERFLOW;
+  }
+  return HSMC_OK;
+}
+
+}  // namespace hsmc
+
+namespace {
+
+__global__ void k_scale(double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
+                        int n, double sf, double lx, double ly, double lz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  // moves.c:135-140: part_conf *= sf, then apply_pbc with the new box
+  x[i] = wrap_once(x[i] * sf, lx);
+  y[i] = wrap_once(y[i] * sf, ly);
+  z[i] = wrap_once(z[i] * sf, lz);
+}
+
+bool valid_box(double L) { return L > 2.0 && std::isfinite(L); }
+
+}  // namespace
+
+extern "C" {
+
+const char *hsmc_gpu_strerror(int code) {
+  switch (code) {
+    case HSMC_OK: return "ok";
+    case HSMC_ERR_CELL_OVERFLOW: return "more particles in one cell than it holds";
+    case HSMC_ERR_CELL_TOO_SMALL: return "size of cells in cell list is smaller than the size of the particles";
+    case HSMC_ERR_INIT_OVERLAP: return "overlap in the initial configuration";
+    case HSMC_ERR_HIP: return "HIP runtime error";
+    case HSMC_ERR_BAD_ARG: return "argument out of range";
+    case HSMC_ERR_NO_MEMORY: return "allocation failed";
+    case HSMC_ERR_NOT_SUPPORTED: return "not supported";
+    case HSMC_ERR_NO_DEVICE: return "no HIP device";
+    default: return "unknown status";
+  }
+}
+
+const char *hsmc_gpu_last_error(const hsmc_gpu_ctx *ctx) {
+  return ctx ? ctx->last_error.c_str() : "";
+}
+
+int hsmc_gpu_create(hsmc_gpu_ctx **out, int device, int n_replicas,
+                    const hsmc_gpu_replica *replicas) {
+  if (!out || n_replicas <= 0 || !replicas) return HSMC_ERR_BAD_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return HSMC_ERR_NO_DEVICE;
+  if (device < 0 || device >= ndev) return HSMC_ERR_BAD_ARG;
+  long ptot = 0;
+  for (int r = 0; r < n_replicas; ++r) {
+    const hsmc_gpu_replica &p = replicas[r];
+    if (p.n_part <= 0 || !valid_box(p.lx) || !valid_box(p.ly) || !valid_box(p.lz) ||
+        !(p.dr_max >= 0.0))
+      return HSMC_ERR_BAD_ARG;
+    ptot += p.n_part;
+  }
+  if (ptot > (1L << 30)) return HSMC_ERR_NOT_SUPPORTED;
+  hsmc_gpu_ctx *ctx = new hsmc_gpu_ctx();
+  ctx->device = device;
+  ctx->R = n_replicas;
+  ctx->reps.resize(n_replicas);
+  std::vector<int> prep(ptot);
+  long poff = 0;
+  for (int r = 0; r < n_replicas; ++r) {
+    ctx->reps[r].p = replicas[r];
+    ctx->reps[r].poff = poff;
+    for (int i = 0; i < replicas[r].n_part; ++i) prep[poff + i] = r;
+    poff += replicas[r].n_part;
+  }
+  ctx->Ptot = ptot;
+  int st = HSMC_OK;
+  auto fail = [&](int code) {
+    hsmc_gpu_destroy(ctx);
+    return code;
+  };
+  if (hipSetDevice(device) != hipSuccess) return fail(HSMC_ERR_HIP);
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(HSMC_ERR_HIP);
+  if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess)
+    return fail(HSMC_ERR_HIP);
+  if (hipMalloc(&ctx->d_reps, sizeof(RepDev) * n_replicas) != hipSuccess ||
+      hipMalloc(&ctx->d_aoff, sizeof(int) * (n_replicas + 1)) != hipSuccess ||
+      hipMalloc(&ctx->d_prep, sizeof(int) * ptot) != hipSuccess ||
+      hipMalloc(&ctx->d_x, sizeof(double) * ptot) != hipSuccess ||
+      hipMalloc(&ctx->d_y, sizeof(double) * ptot) != hipSuccess ||
+      hipMalloc(&ctx->d_z, sizeof(double) * ptot) != hipSuccess ||
+      hipMalloc(&ctx->d_sweep, sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&ctx->d_ctr, sizeof(unsigned long long) * n_replicas * 3 * kSpread) != hipSuccess ||
+      hipMalloc(&ctx->d_err, sizeof(unsigned int)) != hipSuccess)
+    return fail(HSMC_ERR_NO_MEMORY);
+  if (hipMemcpy(ctx->d_prep, prep.data(), sizeof(int) * ptot, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(ctx->d_sweep, 0, sizeof(unsigned long long)) != hipSuccess ||
+      hipMemset(ctx->d_ctr, 0, sizeof(unsigned long long) * n_replicas * 3 * kSpread) != hipSuccess ||
+      hipMemset(ctx->d_err, 0, sizeof(unsigned int)) != hipSuccess ||
+      hipMemset(ctx->d_x, 0, sizeof(double) * ptot) != hipSuccess ||
+      hipMemset(ctx->d_y, 0, sizeof(double) * ptot) != hipSuccess ||
+      hipMemset(ctx->d_z, 0, sizeof(double) * ptot) != hipSuccess)
+    return fail(HSMC_ERR_HIP);
+  st = layout(ctx);
+  if (st) return fail(st);
+  *out = ctx;
+  return HSMC_OK;
+}
+
+int hsmc_gpu_destroy(hsmc_gpu_ctx *ctx) {
+  if (!ctx) return HSMC_OK;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  drop_graph(ctx);
+  void *bufs[] = {ctx->d_reps, ctx->d_aoff, ctx->d_prep, ctx->d_x, ctx->d_y, ctx->d_z,
+                  ctx->d_cnt, ctx->d_spid, ctx->d_sx, ctx->d_sy, ctx->d_sz, ctx->d_sweep,
+                  ctx->d_ctr, ctx->d_err};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  for (int i = 0; i < hsmc_gpu_ctx::kScratch; ++i)
+    if (ctx->scratch[i]) (void)hipFree(ctx->scratch[i]);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return HSMC_OK;
+}
+
+int hsmc_gpu_sync(hsmc_gpu_ctx *ctx) {
+  if (!ctx) return HSMC_ERR_BAD_ARG;
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return check_device_errors(ctx);
+}
+
+int hsmc_gpu_upload(hsmc_gpu_ctx *ctx, int replica, const double *conf4) {
+  if (!ctx || replica < 0 || replica >= ctx->R || !conf4) return HSMC_ERR_BAD_ARG;
+  const RepHost &h = ctx->reps[replica];
+  const int n = h.p.n_part;
+  std::vector<double> x(n), y(n), z(n);
+  for (int i = 0; i < n; ++i) {
+    x[i] = conf4[4 * (size_t)i + 1];
+    y[i] = conf4[4 * (size_t)i + 2];
+    z[i] = conf4[4 * (size_t)i + 3];
+    if (!(x[i] >= 0.0 && x[i] <= h.p.lx && y[i] >= 0.0 && y[i] <= h.p.ly && z[i] >= 0.0 &&
+          z[i] <= h.p.lz)) {
+      ctx->last_error = "particle outside [0, L] (apply_pbc range)";
+      return HSMC_ERR_BAD_ARG;
+    }
+  }
+  HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_x + h.poff, x.data(), sizeof(double) * n,
+                               hipMemcpyHostToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_y + h.poff, y.data(), sizeof(double) * n,
+                               hipMemcpyHostToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_z + h.poff, z.data(), sizeof(double) * n,
+                               hipMemcpyHostToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->version++;
+  return HSMC_OK;
+}
+
+int hsmc_gpu_download(hsmc_gpu_ctx *ctx, int replica, double *conf4) {
+  if (!ctx || replica < 0 || replica >= ctx->R || !conf4) return HSMC_ERR_BAD_ARG;
+  const RepHost &h = ctx->reps[replica];
+  const int n = h.p.n_part;
+  std::vector<double> x(n), y(n), z(n);
+  HSMC_HIP(ctx, hipMemcpyAsync(x.data(), ctx->d_x + h.poff, sizeof(double) * n,
+                               hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(y.data(), ctx->d_y + h.poff, sizeof(double) * n,
+                               hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(z.data(), ctx->d_z + h.poff, sizeof(double) * n,
+                               hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < n; ++i) {
+    conf4[4 * (size_t)i] = (double)i;
+    conf4[4 * (size_t)i + 1] = x[i];
+    conf4[4 * (size_t)i + 2] = y[i];
+    conf4[4 * (size_t)i + 3] = z[i];
+  }
+  return check_device_errors(ctx);
+}
+
+int hsmc_gpu_set_box(hsmc_gpu_ctx *ctx, int replica, double lx, double ly, double lz) {
+  if (!ctx || replica < 0 || replica >= ctx->R) return HSMC_ERR_BAD_ARG;
+  if (!valid_box(lx) || !valid_box(ly) || !valid_box(lz)) return HSMC_ERR_BAD_ARG;
+  hsmc_gpu_replica &p = ctx->reps[replica].p;
+  p.lx = lx; p.ly = ly; p.lz = lz;
+  return layout(ctx);
+}
+
+int hsmc_gpu_scale(hsmc_gpu_ctx *ctx, int replica, double sf, double lx, double ly, double lz) {
+  if (!ctx || replica < 0 || replica >= ctx->R || !(sf > 0.0)) return HSMC_ERR_BAD_ARG;
+  if (!valid_box(lx) || !valid_box(ly) || !valid_box(lz)) return HSMC_ERR_BAD_ARG;
+  RepHost &h = ctx->reps[replica];
+  const int n = h.p.n_part;
+  hipLaunchKernelGGL(k_scale, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
+                     ctx->d_x + h.poff, ctx->d_y + h.poff, ctx->d_z + h.poff, n, sf, lx, ly, lz);
+  HSMC_HIP(ctx, hipGetLastError());
+  h.p.lx = lx; h.p.ly = ly; h.p.lz = lz;
+  return layout(ctx);
+}
+
+int hsmc_gpu_grid(const hsmc_gpu_ctx *ctx, int replica, int32_t out_g[3]) {
+  if (!ctx || replica < 0 || replica >= ctx->R || !out_g) return HSMC_ERR_BAD_ARG;
+  out_g[0] = ctx->reps[replica].gx;
+  out_g[1] = ctx->reps[replica].gy;
+  out_g[2] = ctx->reps[replica].gz;
+  return HSMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,126 @@
+// device_common.h -- FP64 distance arithmetic, cell indexing and the
+// Philox4x32-10 counter RNG shared by every kernel of libhsmc_gpu.
+//
+// All FP64 expressions here are evaluated without contraction (the TU is
+// built with -ffp-contract=off and the pragma below), in the reference's
+// order (moves.c:400-431), so every overlap decision and histogram bin is
+// rounded exactly as the reference rounds it.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace hsmc {
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): 10 rounds of the
+// multiply-xor bijection with the Weyl key schedule.
+__host__ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
+                                                       uint32_t c3, uint32_t k0, uint32_t k1,
+                                                       uint32_t out[4]) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c0 = n0;
+    c1 = (uint32_t)p1;
+    c2 = n2;
+    c3 = (uint32_t)p0;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+// rng_get_double (rng.c:29-31): a 32-bit draw divided by 2^32-1, in [0,1].
+__host__ __device__ __forceinline__ double u32_unit_closed(uint32_t u) {
+  return (double)u / 4294967295.0;
+}
+
+// Minimum image of one component (moves.c:419-424).
+__host__ __device__ __forceinline__ double min_image(double d, double L, double half) {
+  if (d > half)
+    d -= L;
+  else if (d < -half)
+    d += L;
+  return d;
+}
+
+// Squared distance of the reference's compute_dist with unit scale
+// (moves.c:414-427): a - b per axis, minimum image, dx*dx+dy*dy+dz*dz.
+// dr < 1 <=> dr^2 < 1 for correctly rounded sqrt, so overlap tests use it.
+__host__ __device__ __forceinline__ double dist2(double ax, double ay, double az, double bx,
+                                                 double by, double bz, double lx, double ly,
+                                                 double lz, double hx, double hy, double hz) {
+  const double dx = min_image(ax - bx, lx, hx);
+  const double dy = min_image(ay - by, ly, hy);
+  const double dz = min_image(az - bz, lz, hz);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// compute_dist with a scale factor (moves.c:405-427): dx = (a-b)*sf and a
+// box of L*sf, half box (L*sf)/2.
+__host__ __device__ __forceinline__ double dist2_scaled(double ax, double ay, double az,
+                                                        double bx, double by, double bz,
+                                                        double lx, double ly, double lz,
+                                                        double sf) {
+  const double Lx = lx * sf, Ly = ly * sf, Lz = lz * sf;
+  const double hx = Lx / 2.0, hy = Ly / 2.0, hz = Lz / 2.0;
+  const double dx = min_image((ax - bx) * sf, Lx, hx);
+  const double dy = min_image((ay - by) * sf, Ly, hy);
+  const double dz = min_image((az - bz) * sf, Lz, hz);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// apply_pbc for one coordinate (moves.c:215-226): a single wrap.
+__host__ __device__ __forceinline__ double wrap_once(double x, double L) {
+  if (x > L)
+    x -= L;
+  else if (x < 0.0)
+    x += L;
+  return x;
+}
+
+// Cell of coordinate x on a grid of g cells of side `side` whose origin is
+// shifted by `shift` in [0, side): t = x - shift (wrapped), floor(t/side),
+// clamped to g-1 for t == L.
+__host__ __device__ __forceinline__ int shifted_cell(double x, double shift, double L,
+                                                     double side, int g) {
+  double t = x - shift;
+  if (t < 0.0) t += L;
+  int c = (int)(t / side);
+  if (c >= g) c = g - 1;
+  return c;
+}
+
+// Cell of coordinate x on an unshifted grid (cell_part_idx, cell_list.c:244-246),
+// clamped into [0, g-1].
+__host__ __device__ __forceinline__ int plain_cell(double x, double side, int g) {
+  int c = (int)(x / side);
+  if (c >= g) c = g - 1;
+  if (c < 0) c = 0;
+  return c;
+}
+
+// Per-axis neighbour offsets of the 27-cell stencil that are distinct on a
+// periodic grid of g cells: g >= 3 -> {-1,0,1}; g == 2 -> {0,1}; g == 1 -> {0}.
+__host__ __device__ __forceinline__ bool stencil_axis_ok(int d, int g) {
+  if (g >= 3) return true;
+  if (g == 2) return d != -1;
+  return d == 0;
+}
+
+__host__ __device__ __forceinline__ int wrap_index(int i, int g) {
+  if (i < 0) i += g;
+  else if (i >= g) i -= g;
+  return i;
+}
+
+}  // namespace hsmc

@@ -1,0 +1,601 @@
+// observables.hip -- cell list, overlap tests, pair histograms and Widom
+// insertions on one replica's configuration.
+//
+// Every neighbour search runs on an "observable grid": an unshifted cell grid
+// whose side is at least the cutoff of the question asked (1 for overlaps and
+// Widom, 1/sf for a compression by sf, rmax for the histograms), stored in
+// CSR form (cell start offsets, particle ids and FP64 positions sorted by
+// cell).  Because the side covers the cutoff, the set of pairs examined is
+// every pair within the cutoff, which is also the set the reference examines
+// through its own cell list whenever its cells are at least that wide
+// (compute_press.c:61-69 enforces it for the contact histogram).
+#include <math.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "ctx.h"
+#include "device_common.h"
+
+using namespace hsmc;
+
+namespace {
+
+enum Slot { S_CNT = 0, S_START, S_CELLOF, S_IDS, S_PX, S_PY, S_PZ, S_FILL, S_Q0, S_Q1, S_Q2, S_OUT0, S_OUT1 };
+
+struct Grid {
+  int nx, ny, nz;
+  double sx, sy, sz;
+  double lx, ly, lz, hx, hy, hz;
+  const int *start;   // C + 1
+  const int *ids;     // N, particle id by sorted position
+  const double *px, *py, *pz;
+};
+
+__global__ void k_grid_count(const double *__restrict__ x, const double *__restrict__ y,
+                             const double *__restrict__ z, int n, int nx, int ny, int nz, double sx,
+                             double sy, double sz, int *__restrict__ cnt, int *__restrict__ cellof) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = (plain_cell(x[i], sx, nx) * ny + plain_cell(y[i], sy, ny)) * nz + plain_cell(z[i], sz, nz);
+  cellof[i] = c;
+  atomicAdd(&cnt[c], 1);
+}
+
+// Exclusive prefix sum of cnt[0..n) into start[0..n], one workgroup.
+__global__ void __launch_bounds__(1024) k_scan1(const int *__restrict__ cnt, int n, int *__restrict__ start) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int b = t * per, e = min(n, b + per);
+  int s = 0;
+  for (int i = b; i < e; ++i) s += cnt[i];
+  part[t] = s;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    int v = (t >= d) ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+  for (int i = b; i < e; ++i) {
+    start[i] = run;
+    run += cnt[i];
+  }
+  if (t == 1023) start[n] = part[1023];
+}
+
+__global__ void k_grid_scatter(const double *__restrict__ x, const double *__restrict__ y,
+                               const double *__restrict__ z, int n, const int *__restrict__ cellof,
+                               const int *__restrict__ start, int *__restrict__ fill,
+                               int *__restrict__ ids, double *__restrict__ px,
+                               double *__restrict__ py, double *__restrict__ pz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cellof[i];
+  const int pos = start[c] + atomicAdd(&fill[c], 1);
+  ids[pos] = i;
+  px[pos] = x[i];
+  py[pos] = y[i];
+  pz[pos] = z[i];
+}
+
+// One thread per cell: ascending ids, then the reference's part_cell row
+// [count, ids..., -1...] (cell_list.c:148-171).
+__global__ void k_part_cell(const int *__restrict__ start, int C, int *__restrict__ ids,
+                            int max_part, int *__restrict__ out, unsigned *__restrict__ err) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  const int b = start[c], n = start[c + 1] - start[c];
+  int *row = out + (size_t)c * max_part;
+  if (n > max_part - 1) {  // cell_list_check (cell_list.c:229-232)
+    atomicOr(err, kErrOverflow);
+    row[0] = n;
+    return;
+  }
+  for (int i = 1; i < n; ++i) {  // insertion sort of a few ids
+    const int v = ids[b + i];
+    int j = i - 1;
+    while (j >= 0 && ids[b + j] > v) {
+      ids[b + j + 1] = ids[b + j];
+      --j;
+    }
+    ids[b + j + 1] = v;
+  }
+  row[0] = n;
+  for (int j = 1; j < max_part; ++j) row[j] = (j <= n) ? ids[b + j - 1] : -1;
+}
+
+// Iterate the distinct cells of the 27-stencil around (cx,cy,cz).
+template <class F>
+__device__ __forceinline__ bool for_stencil(const Grid &g, int cx, int cy, int cz, F &&f) {
+  for (int dx = -1; dx <= 1; ++dx) {
+    if (!stencil_axis_ok(dx, g.nx)) continue;
+    const int ax = wrap_index(cx + dx, g.nx);
+    for (int dy = -1; dy <= 1; ++dy) {
+      if (!stencil_axis_ok(dy, g.ny)) continue;
+      const int ay = wrap_index(cy + dy, g.ny);
+      for (int dz = -1; dz <= 1; ++dz) {
+        if (!stencil_axis_ok(dz, g.nz)) continue;
+        const int c = (ax * g.ny + ay) * g.nz + wrap_index(cz + dz, g.nz);
+        if (f(g.start[c], g.start[c + 1])) return true;
+      }
+    }
+  }
+  return false;
+}
+
+__device__ __forceinline__ bool point_overlaps(const Grid &g, double qx, double qy, double qz,
+                                               int skip_id) {
+  const int cx = plain_cell(qx, g.sx, g.nx), cy = plain_cell(qy, g.sy, g.ny),
+            cz = plain_cell(qz, g.sz, g.nz);
+  return for_stencil(g, cx, cy, cz, [&](int b, int e) {
+    for (int p = b; p < e; ++p) {
+      if (g.ids[p] == skip_id) continue;
+      if (dist2(qx, qy, qz, g.px[p], g.py[p], g.pz[p], g.lx, g.ly, g.lz, g.hx, g.hy, g.hz) < 1.0)
+        return true;
+    }
+    return false;
+  });
+}
+
+__global__ void k_overlap_batch(Grid g, const int *__restrict__ idx, const double *__restrict__ q,
+                                int m, unsigned char *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  out[k] = point_overlaps(g, q[3 * k], q[3 * k + 1], q[3 * k + 2], idx[k]) ? 1 : 0;
+}
+
+__global__ void k_widom_points(Grid g, const double *__restrict__ q, int m,
+                               unsigned char *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  out[k] = point_overlaps(g, q[3 * k], q[3 * k + 1], q[3 * k + 2], -1) ? 0 : 1;
+}
+
+__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v) {
+  __shared__ unsigned long long red[16];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  unsigned long long s = 0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < (int)((blockDim.x + 63) >> 6); ++i) s += red[i];
+  }
+  return s;
+}
+
+// widom_insertion (compute_widom_chem_pot.c:44-71) on a Philox stream:
+// point i has counter {i_lo, i_hi, sample_lo, 0x5749xxxx}, coordinates
+// u/(2^32-1) * L (widom_rand_pos, :73-80).
+__global__ void __launch_bounds__(256)
+k_widom_uniform(Grid g, uint32_t k0, uint32_t k1, unsigned long long sample, long long first,
+                long long n, unsigned long long *__restrict__ hits) {
+  unsigned long long h = 0;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const uint32_t c3 = 0x57490000u | (uint32_t)((sample >> 32) & 0xFFFFu);
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+    const unsigned long long i = (unsigned long long)(first + t);
+    uint32_t u[4];
+    philox4x32_10((uint32_t)i, (uint32_t)(i >> 32), (uint32_t)sample, c3, k0, k1, u);
+    const double qx = u32_unit_closed(u[0]) * g.lx;
+    const double qy = u32_unit_closed(u[1]) * g.ly;
+    const double qz = u32_unit_closed(u[2]) * g.lz;
+    if (!point_overlaps(g, qx, qy, qz, -1)) ++h;
+  }
+  const unsigned long long s = block_sum_u64(h);
+  if (threadIdx.x == 0 && s) atomicAdd(hits, s);
+}
+
+// check_overlap(idx, sf, sf, sf) (moves.c:157-212) for each query.
+__global__ void k_overlap_scaled(Grid g, const double *__restrict__ x, const double *__restrict__ y,
+                                 const double *__restrict__ z, const int *__restrict__ idx,
+                                 const double *__restrict__ sfv, int m,
+                                 unsigned char *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const int i = idx[k];
+  const double sf = sfv[k];
+  const double ax = x[i], ay = y[i], az = z[i];
+  const int cx = plain_cell(ax, g.sx, g.nx), cy = plain_cell(ay, g.sy, g.ny), cz = plain_cell(az, g.sz, g.nz);
+  const bool ov = for_stencil(g, cx, cy, cz, [&](int b, int e) {
+    for (int p = b; p < e; ++p) {
+      if (g.ids[p] == i) continue;
+      if (dist2_scaled(ax, ay, az, g.px[p], g.py[p], g.pz[p], g.lx, g.ly, g.lz, sf) < 1.0)
+        return true;
+    }
+    return false;
+  });
+  out[k] = ov ? 1 : 0;
+}
+
+// Any pair overlapping after scaling by sf[k]: bit k of *mask.  Pairs whose
+// unscaled squared distance exceeds cut2 cannot overlap for any sf[k].
+__global__ void __launch_bounds__(256)
+k_any_overlap_scaled(Grid g, int n, const double *__restrict__ sfv, int nsf, double cut2,
+                     unsigned long long *__restrict__ mask) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double ax = g.px[p], ay = g.py[p], az = g.pz[p];
+  const int cx = plain_cell(ax, g.sx, g.nx), cy = plain_cell(ay, g.sy, g.ny), cz = plain_cell(az, g.sz, g.nz);
+  unsigned long long m = 0;
+  for_stencil(g, cx, cy, cz, [&](int b, int e) {
+    for (int q = b; q < e; ++q) {
+      if (q <= p) continue;  // each unordered pair once
+      const double s0 = dist2(ax, ay, az, g.px[q], g.py[q], g.pz[q], g.lx, g.ly, g.lz, g.hx, g.hy, g.hz);
+      if (s0 >= cut2) continue;
+      for (int k = 0; k < nsf; ++k)
+        if (dist2_scaled(ax, ay, az, g.px[q], g.py[q], g.pz[q], g.lx, g.ly, g.lz, sfv[k]) < 1.0)
+          m |= (1ull << k);
+    }
+    return false;
+  });
+  if (m) atomicOr(mask, m);
+}
+
+__global__ void k_pair_dist(const double *__restrict__ x, const double *__restrict__ y,
+                            const double *__restrict__ z, const int *__restrict__ ii,
+                            const int *__restrict__ jj, int m, double sf, double lx, double ly,
+                            double lz, double *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const int i = ii[k], j = jj[k];
+  out[k] = sqrt(dist2_scaled(x[i], y[i], z[i], x[j], y[j], z[j], lx, ly, lz, sf));
+}
+
+// Pair histogram: every unordered pair with dr < rmax_eff, bin
+// (int)((dr - 1.0) / dr_bin) (compute_rdf.c:118-124, compute_press.c:150-155).
+__global__ void __launch_bounds__(256)
+k_pair_hist(Grid g, int n, double dr_bin, int nbins, double rmax_eff, double cut2,
+            unsigned long long *__restrict__ counts, unsigned long long *__restrict__ oor) {
+  extern __shared__ unsigned int hist[];
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) hist[b] = 0;
+  __syncthreads();
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long bad = 0;
+  if (p < n) {
+    const double ax = g.px[p], ay = g.py[p], az = g.pz[p];
+    const int cx = plain_cell(ax, g.sx, g.nx), cy = plain_cell(ay, g.sy, g.ny), cz = plain_cell(az, g.sz, g.nz);
+    for_stencil(g, cx, cy, cz, [&](int b, int e) {
+      for (int q = max(b, p + 1); q < e; ++q) {
+        const double s = dist2(ax, ay, az, g.px[q], g.py[q], g.pz[q], g.lx, g.ly, g.lz, g.hx, g.hy, g.hz);
+        if (s >= cut2) continue;
+        const double dr = sqrt(s);
+        if (dr < rmax_eff) {
+          const int bin = (int)((dr - 1.0) / dr_bin);
+          if (bin >= 0 && bin < nbins)
+            atomicAdd(&hist[bin], 1u);
+          else
+            ++bad;
+        }
+      }
+      return false;
+    });
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+    if (hist[b]) atomicAdd(&counts[b], (unsigned long long)hist[b]);
+  const unsigned long long s = block_sum_u64(bad);
+  if (threadIdx.x == 0 && s) atomicAdd(oor, s);
+}
+
+int grid_cells_for(double L, double cut) {
+  int n = (int)floor(L / cut);
+  if (n < 1) n = 1;
+  while (n > 1 && L / n < cut) --n;
+  return n;
+}
+
+// Build (or reuse) the observable grid of `replica` for cutoff `cut`.
+int build_grid(hsmc_gpu_ctx *ctx, int replica, double cut, Grid &g) {
+  const RepHost &h = ctx->reps[replica];
+  const int N = h.p.n_part;
+  const int nx = grid_cells_for(h.p.lx, cut), ny = grid_cells_for(h.p.ly, cut),
+            nz = grid_cells_for(h.p.lz, cut);
+  const long C = (long)nx * ny * nz;
+  if (C > (1L << 28)) return HSMC_ERR_NOT_SUPPORTED;
+  int *cnt = (int *)scratch(ctx, S_CNT, sizeof(int) * C);
+  int *fill = (int *)scratch(ctx, S_FILL, sizeof(int) * C);
+  int *start = (int *)scratch(ctx, S_START, sizeof(int) * (C + 1));
+  int *cellof = (int *)scratch(ctx, S_CELLOF, sizeof(int) * N);
+  int *ids = (int *)scratch(ctx, S_IDS, sizeof(int) * N);
+  double *px = (double *)scratch(ctx, S_PX, sizeof(double) * N);
+  double *py = (double *)scratch(ctx, S_PY, sizeof(double) * N);
+  double *pz = (double *)scratch(ctx, S_PZ, sizeof(double) * N);
+  if (!cnt || !fill || !start || !cellof || !ids || !px || !py || !pz) return HSMC_ERR_NO_MEMORY;
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  g.lx = h.p.lx; g.ly = h.p.ly; g.lz = h.p.lz;
+  g.hx = h.p.lx / 2.0; g.hy = h.p.ly / 2.0; g.hz = h.p.lz / 2.0;
+  g.sx = h.p.lx / nx; g.sy = h.p.ly / ny; g.sz = h.p.lz / nz;
+  g.start = start; g.ids = ids; g.px = px; g.py = py; g.pz = pz;
+  if (ctx->aux_replica == replica && ctx->aux_version == ctx->version && ctx->aux_n[0] == nx &&
+      ctx->aux_n[1] == ny && ctx->aux_n[2] == nz)
+    return HSMC_OK;
+  const double *x = ctx->d_x + h.poff, *y = ctx->d_y + h.poff, *z = ctx->d_z + h.poff;
+  HSMC_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(int) * C, ctx->stream));
+  HSMC_HIP(ctx, hipMemsetAsync(fill, 0, sizeof(int) * C, ctx->stream));
+  const unsigned blocks = (unsigned)((N + 255) / 256);
+  hipLaunchKernelGGL(k_grid_count, dim3(blocks), dim3(256), 0, ctx->stream, x, y, z, N, nx, ny, nz,
+                     g.sx, g.sy, g.sz, cnt, cellof);
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, ctx->stream, cnt, (int)C, start);
+  hipLaunchKernelGGL(k_grid_scatter, dim3(blocks), dim3(256), 0, ctx->stream, x, y, z, N, cellof,
+                     start, fill, ids, px, py, pz);
+  HSMC_HIP(ctx, hipGetLastError());
+  ctx->aux_replica = replica;
+  ctx->aux_version = ctx->version;
+  ctx->aux_n[0] = nx; ctx->aux_n[1] = ny; ctx->aux_n[2] = nz;
+  ctx->aux_cut = cut;
+  return HSMC_OK;
+}
+
+bool bad_replica(const hsmc_gpu_ctx *ctx, int r) { return !ctx || r < 0 || r >= ctx->R; }
+
+}  // namespace
+
+extern "C" {
+
+int hsmc_gpu_cell_list(hsmc_gpu_ctx *ctx, int replica, int nx, int ny, int nz, int max_part,
+                       int32_t *part_cell, int32_t *neigh_mat) {
+  if (bad_replica(ctx, replica) || nx < 1 || ny < 1 || nz < 1 || max_part < 2)
+    return HSMC_ERR_BAD_ARG;
+  const RepHost &h = ctx->reps[replica];
+  const double sx = h.p.lx / nx, sy = h.p.ly / ny, sz = h.p.lz / nz;
+  if (sx < 1.0 || sy < 1.0 || sz < 1.0) return HSMC_ERR_CELL_TOO_SMALL;  // cell_list.c:126-129
+  const long C = (long)nx * ny * nz;
+  const int N = h.p.n_part;
+  if (part_cell) {
+    int *cnt = (int *)scratch(ctx, S_CNT, sizeof(int) * C);
+    int *fill = (int *)scratch(ctx, S_FILL, sizeof(int) * C);
+    int *start = (int *)scratch(ctx, S_START, sizeof(int) * (C + 1));
+    int *cellof = (int *)scratch(ctx, S_CELLOF, sizeof(int) * N);
+    int *ids = (int *)scratch(ctx, S_IDS, sizeof(int) * N);
+    double *px = (double *)scratch(ctx, S_PX, sizeof(double) * N);
+    double *py = (double *)scratch(ctx, S_PY, sizeof(double) * N);
+    double *pz = (double *)scratch(ctx, S_PZ, sizeof(double) * N);
+    int *rows = (int *)scratch(ctx, S_OUT0, sizeof(int) * C * max_part);
+    if (!cnt || !fill || !start || !cellof || !ids || !px || !py || !pz || !rows)
+      return HSMC_ERR_NO_MEMORY;
+    ctx->aux_replica = -1;  // the CSR scratch now holds this grid
+    const double *x = ctx->d_x + h.poff, *y = ctx->d_y + h.poff, *z = ctx->d_z + h.poff;
+    HSMC_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(int) * C, ctx->stream));
+    HSMC_HIP(ctx, hipMemsetAsync(fill, 0, sizeof(int) * C, ctx->stream));
+    const unsigned blocks = (unsigned)((N + 255) / 256);
+    hipLaunchKernelGGL(k_grid_count, dim3(blocks), dim3(256), 0, ctx->stream, x, y, z, N, nx, ny,
+                       nz, sx, sy, sz, cnt, cellof);
+    hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, ctx->stream, cnt, (int)C, start);
+    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks), dim3(256), 0, ctx->stream, x, y, z, N, cellof,
+                       start, fill, ids, px, py, pz);
+    hipLaunchKernelGGL(k_part_cell, dim3((unsigned)((C + 255) / 256)), dim3(256), 0, ctx->stream,
+                       start, (int)C, ids, max_part, rows, ctx->d_err);
+    HSMC_HIP(ctx, hipGetLastError());
+    HSMC_HIP(ctx, hipMemcpyAsync(part_cell, rows, sizeof(int) * C * max_part,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int st = check_device_errors(ctx);
+    if (st) return st;
+  }
+  if (neigh_mat) {  // neigh_id (cell_list.c:266-301), host-side table
+    for (int a = 0; a < nx; ++a)
+      for (int b = 0; b < ny; ++b)
+        for (int c = 0; c < nz; ++c) {
+          const long ref = ((long)a * ny + b) * nz + c;
+          int counter = 0;
+          for (int i = -1; i < 2; ++i)
+            for (int j = -1; j < 2; ++j)
+              for (int k = -1; k < 2; ++k) {
+                int ix = a + i, iy = b + j, iz = c + k;
+                if (ix > nx - 1) ix -= nx; else if (ix < 0) ix += nx;
+                if (iy > ny - 1) iy -= ny; else if (iy < 0) iy += ny;
+                if (iz > nz - 1) iz -= nz; else if (iz < 0) iz += nz;
+                neigh_mat[ref * 27 + counter] = (ix * ny + iy) * nz + iz;
+                ++counter;
+              }
+        }
+  }
+  return HSMC_OK;
+}
+
+int hsmc_gpu_overlap_batch(hsmc_gpu_ctx *ctx, int replica, const int32_t *idx, const double *xyz,
+                           int m, uint8_t *out) {
+  if (bad_replica(ctx, replica) || m < 0 || (m > 0 && (!idx || !xyz || !out))) return HSMC_ERR_BAD_ARG;
+  if (m == 0) return HSMC_OK;
+  for (int k = 0; k < m; ++k)
+    if (idx[k] < 0 || idx[k] >= ctx->reps[replica].p.n_part) return HSMC_ERR_BAD_ARG;
+  Grid g;
+  int st = build_grid(ctx, replica, 1.0 + 1e-9, g);
+  if (st) return st;
+  int *didx = (int *)scratch(ctx, S_Q0, sizeof(int) * m);
+  double *dq = (double *)scratch(ctx, S_Q1, sizeof(double) * 3 * m);
+  unsigned char *dout = (unsigned char *)scratch(ctx, S_Q2, m);
+  if (!didx || !dq || !dout) return HSMC_ERR_NO_MEMORY;
+  HSMC_HIP(ctx, hipMemcpyAsync(didx, idx, sizeof(int) * m, hipMemcpyHostToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(dq, xyz, sizeof(double) * 3 * m, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_overlap_batch, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, g, didx, dq,
+                     m, dout);
+  HSMC_HIP(ctx, hipGetLastError());
+  HSMC_HIP(ctx, hipMemcpyAsync(out, dout, m, hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HSMC_OK;
+}
+
+int hsmc_gpu_overlap_scaled(hsmc_gpu_ctx *ctx, int replica, const int32_t *idx, const double *sf,
+                            int m, uint8_t *out) {
+  if (bad_replica(ctx, replica) || m < 0 || (m > 0 && (!idx || !sf || !out))) return HSMC_ERR_BAD_ARG;
+  if (m == 0) return HSMC_OK;
+  double smin = 1.0;
+  for (int k = 0; k < m; ++k) {
+    if (idx[k] < 0 || idx[k] >= ctx->reps[replica].p.n_part || !(sf[k] > 0.0)) return HSMC_ERR_BAD_ARG;
+    smin = std::min(smin, sf[k]);
+  }
+  Grid g;
+  int st = build_grid(ctx, replica, (1.0 / smin) * (1.0 + 1e-9), g);
+  if (st) return st;
+  const RepHost &h = ctx->reps[replica];
+  int *didx = (int *)scratch(ctx, S_Q0, sizeof(int) * m);
+  double *dsf = (double *)scratch(ctx, S_Q1, sizeof(double) * m);
+  unsigned char *dout = (unsigned char *)scratch(ctx, S_Q2, m);
+  if (!didx || !dsf || !dout) return HSMC_ERR_NO_MEMORY;
+  HSMC_HIP(ctx, hipMemcpyAsync(didx, idx, sizeof(int) * m, hipMemcpyHostToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(dsf, sf, sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_overlap_scaled, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, g,
+                     ctx->d_x + h.poff, ctx->d_y + h.poff, ctx->d_z + h.poff, didx, dsf, m, dout);
+  HSMC_HIP(ctx, hipGetLastError());
+  HSMC_HIP(ctx, hipMemcpyAsync(out, dout, m, hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HSMC_OK;
+}
+
+int hsmc_gpu_any_overlap_scaled(hsmc_gpu_ctx *ctx, int replica, const double *sf, int n_sf,
+                                uint8_t *no_overlap) {
+  if (bad_replica(ctx, replica) || n_sf < 1 || n_sf > 64 || !sf || !no_overlap) return HSMC_ERR_BAD_ARG;
+  double smin = 1.0;
+  for (int k = 0; k < n_sf; ++k) {
+    if (!(sf[k] > 0.0)) return HSMC_ERR_BAD_ARG;
+    smin = std::min(smin, sf[k]);
+  }
+  const double cut = (1.0 / smin) * (1.0 + 1e-6);
+  Grid g;
+  int st = build_grid(ctx, replica, cut, g);
+  if (st) return st;
+  const int N = ctx->reps[replica].p.n_part;
+  double *dsf = (double *)scratch(ctx, S_Q1, sizeof(double) * n_sf);
+  unsigned long long *dmask = (unsigned long long *)scratch(ctx, S_OUT1, sizeof(unsigned long long));
+  if (!dsf || !dmask) return HSMC_ERR_NO_MEMORY;
+  HSMC_HIP(ctx, hipMemcpyAsync(dsf, sf, sizeof(double) * n_sf, hipMemcpyHostToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemsetAsync(dmask, 0, sizeof(unsigned long long), ctx->stream));
+  hipLaunchKernelGGL(k_any_overlap_scaled, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, g, N,
+                     dsf, n_sf, cut * cut, dmask);
+  HSMC_HIP(ctx, hipGetLastError());
+  unsigned long long mask = 0;
+  HSMC_HIP(ctx, hipMemcpyAsync(&mask, dmask, sizeof(mask), hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < n_sf; ++k) no_overlap[k] = ((mask >> k) & 1ull) ? 0 : 1;
+  return HSMC_OK;
+}
+
+int hsmc_gpu_pair_dist(hsmc_gpu_ctx *ctx, int replica, const int32_t *i, const int32_t *j, int m,
+                       double sf, double *out) {
+  if (bad_replica(ctx, replica) || m < 0 || (m > 0 && (!i || !j || !out)) || !(sf > 0.0))
+    return HSMC_ERR_BAD_ARG;
+  if (m == 0) return HSMC_OK;
+  const RepHost &h = ctx->reps[replica];
+  for (int k = 0; k < m; ++k)
+    if (i[k] < 0 || i[k] >= h.p.n_part || j[k] < 0 || j[k] >= h.p.n_part) return HSMC_ERR_BAD_ARG;
+  int *di = (int *)scratch(ctx, S_Q0, sizeof(int) * m);
+  int *dj = (int *)scratch(ctx, S_Q2, sizeof(int) * m);
+  double *dout = (double *)scratch(ctx, S_Q1, sizeof(double) * m);
+  if (!di || !dj || !dout) return HSMC_ERR_NO_MEMORY;
+  HSMC_HIP(ctx, hipMemcpyAsync(di, i, sizeof(int) * m, hipMemcpyHostToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(dj, j, sizeof(int) * m, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_pair_dist, dim3((m + 255) / 256), dim3(256), 0, ctx->stream,
+                     ctx->d_x + h.poff, ctx->d_y + h.poff, ctx->d_z + h.poff, di, dj, m, sf,
+                     h.p.lx, h.p.ly, h.p.lz, dout);
+  HSMC_HIP(ctx, hipGetLastError());
+  HSMC_HIP(ctx, hipMemcpyAsync(out, dout, sizeof(double) * m, hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HSMC_OK;
+}
+
+static int pair_hist(hsmc_gpu_ctx *ctx, int replica, double dr_bin, int nbins, double rmax_eff,
+                     int64_t *counts, int64_t *oor) {
+  if (bad_replica(ctx, replica) || !(dr_bin > 0.0) || nbins < 1 || nbins > 8192 ||
+      !(rmax_eff > 1.0) || !counts)
+    return HSMC_ERR_BAD_ARG;
+  const RepHost &h = ctx->reps[replica];
+  const double half = std::min(h.p.lx, std::min(h.p.ly, h.p.lz)) / 2.0;
+  if (rmax_eff > half * (1.0 + 1e-12)) {
+    ctx->last_error = "histogram cutoff beyond half the box (minimum image)";
+    return HSMC_ERR_BAD_ARG;
+  }
+  const double cut = rmax_eff * (1.0 + 1e-9);
+  Grid g;
+  int st = build_grid(ctx, replica, cut, g);
+  if (st) return st;
+  const int N = h.p.n_part;
+  unsigned long long *dcounts = (unsigned long long *)scratch(ctx, S_OUT0, sizeof(unsigned long long) * (nbins + 1));
+  if (!dcounts) return HSMC_ERR_NO_MEMORY;
+  HSMC_HIP(ctx, hipMemsetAsync(dcounts, 0, sizeof(unsigned long long) * (nbins + 1), ctx->stream));
+  const double cut2 = (rmax_eff * (1.0 + 1e-6)) * (rmax_eff * (1.0 + 1e-6));
+  hipLaunchKernelGGL(k_pair_hist, dim3((N + 255) / 256), dim3(256), sizeof(unsigned int) * nbins,
+                     ctx->stream, g, N, dr_bin, nbins, rmax_eff, cut2, dcounts, dcounts + nbins);
+  HSMC_HIP(ctx, hipGetLastError());
+  std::vector<unsigned long long> hc(nbins + 1);
+  HSMC_HIP(ctx, hipMemcpyAsync(hc.data(), dcounts, sizeof(unsigned long long) * (nbins + 1),
+                               hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int b = 0; b < nbins; ++b) counts[b] = (int64_t)hc[b];
+  if (oor) *oor = (int64_t)hc[nbins];
+  return HSMC_OK;
+}
+
+int hsmc_gpu_rdf_counts(hsmc_gpu_ctx *ctx, int replica, double dr_bin, int nbins, double rmax_eff,
+                        int64_t *counts, int64_t *out_of_range) {
+  return pair_hist(ctx, replica, dr_bin, nbins, rmax_eff, counts, out_of_range);
+}
+
+int hsmc_gpu_contact_counts(hsmc_gpu_ctx *ctx, int replica, double dr_bin, int nbins,
+                            double rmax_eff, int64_t *counts, int64_t *out_of_range) {
+  return pair_hist(ctx, replica, dr_bin, nbins, rmax_eff, counts, out_of_range);
+}
+
+int hsmc_gpu_widom_points(hsmc_gpu_ctx *ctx, int replica, const double *xyz, int m, uint8_t *hit,
+                          int64_t *hits) {
+  if (bad_replica(ctx, replica) || m < 0 || (m > 0 && !xyz)) return HSMC_ERR_BAD_ARG;
+  if (m == 0) {
+    if (hits) *hits = 0;
+    return HSMC_OK;
+  }
+  Grid g;
+  int st = build_grid(ctx, replica, 1.0 + 1e-9, g);
+  if (st) return st;
+  double *dq = (double *)scratch(ctx, S_Q1, sizeof(double) * 3 * m);
+  unsigned char *dout = (unsigned char *)scratch(ctx, S_Q2, m);
+  if (!dq || !dout) return HSMC_ERR_NO_MEMORY;
+  HSMC_HIP(ctx, hipMemcpyAsync(dq, xyz, sizeof(double) * 3 * m, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_widom_points, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, g, dq, m, dout);
+  HSMC_HIP(ctx, hipGetLastError());
+  std::vector<uint8_t> tmp;
+  uint8_t *dst = hit;
+  if (!dst) {
+    tmp.resize(m);
+    dst = tmp.data();
+  }
+  HSMC_HIP(ctx, hipMemcpyAsync(dst, dout, m, hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (hits) {
+    int64_t s = 0;
+    for (int k = 0; k < m; ++k) s += dst[k];
+    *hits = s;
+  }
+  return HSMC_OK;
+}
+
+int hsmc_gpu_widom_uniform(hsmc_gpu_ctx *ctx, int replica, uint64_t seed, uint64_t sample,
+                           int64_t first, int64_t n, int64_t *hits) {
+  if (bad_replica(ctx, replica) || n < 0 || first < 0 || !hits) return HSMC_ERR_BAD_ARG;
+  *hits = 0;
+  if (n == 0) return HSMC_OK;
+  Grid g;
+  int st = build_grid(ctx, replica, 1.0 + 1e-9, g);
+  if (st) return st;
+  unsigned long long *dh = (unsigned long long *)scratch(ctx, S_OUT1, sizeof(unsigned long long));
+  if (!dh) return HSMC_ERR_NO_MEMORY;
+  HSMC_HIP(ctx, hipMemsetAsync(dh, 0, sizeof(unsigned long long), ctx->stream));
+  const long long per_block = 256LL * 16;
+  long long blocks = (n + per_block - 1) / per_block;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_widom_uniform, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, g,
+                     (uint32_t)seed, (uint32_t)(seed >> 32), (unsigned long long)sample,
+                     (long long)first, (long long)n, dh);
+  HSMC_HIP(ctx, hipGetLastError());
+  unsigned long long hv = 0;
+  HSMC_HIP(ctx, hipMemcpyAsync(&hv, dh, sizeof(hv), hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *hits = (int64_t)hv;
+  return HSMC_OK;
+}
+
+}  // extern "C"

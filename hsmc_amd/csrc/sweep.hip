@@ -1,0 +1,489 @@
+// sweep.hip -- checkerboard cell-parallel Metropolis sweep (the hot path).
+//
+// One sweep of every replica (the work of sweep_nvt, nvt.c:201-209, which
+// makes N part_move attempts, moves.c:27-80):
+//   1. k_rebin: the sweep grid is shifted by a random vector in [0, side)^3
+//      and every particle is filed into its cell (slot order by atomics);
+//   2. k_phase x 8: the 8 sublattices (cells whose three indices have a
+//      fixed parity) are visited in a random order.  Cells of one sublattice
+//      are at least one cell apart, cells are wider than a diameter, and a
+//      move that leaves its cell is rejected, so the active cells of a phase
+//      never interact and run concurrently.  An active cell makes as many
+//      attempts as it has occupants, each on an occupant drawn uniformly
+//      (rng_get_int semantics) and displaced by (r - 0.5) * dr_max per axis
+//      (moves.c:52-54), wrapped by apply_pbc (moves.c:215-226) and tested
+//      for overlap with every other sphere in the 27 neighbouring cells with
+//      the reference's FP64 arithmetic (moves.c:400-431).
+// The random numbers come from Philox4x32-10 keyed by the replica seed with
+// counters (cell, attempt, sweep), so a run is reproducible bit for bit and
+// equals the CPU restatement in the oracle.
+#include <stdio.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "ctx.h"
+#include "device_common.h"
+
+using namespace hsmc;
+
+namespace {
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Shift of the sweep grid for one sweep: 3 uniform draws in [0, side).
+__device__ __forceinline__ void sweep_shift(const RepDev &rp, unsigned long long sweep,
+                                            double &shx, double &shy, double &shz) {
+  uint32_t u[4];
+  philox4x32_10(0xFFFFFFFFu, 0u, (uint32_t)sweep, (uint32_t)(sweep >> 32), rp.k0, rp.k1, u);
+  shx = ((double)u[0] * 0x1p-32) * rp.sx;
+  shy = ((double)u[1] * 0x1p-32) * rp.sy;
+  shz = ((double)u[2] * 0x1p-32) * rp.sz;
+}
+
+// Sublattice visited in phase `phase`: a Fisher-Yates order of 0..7 drawn
+// from one Philox block, kept as 8 nibbles of a register.
+__device__ __forceinline__ int sweep_colour(const RepDev &rp, unsigned long long sweep,
+                                            int phase) {
+  uint32_t u[4];
+  philox4x32_10(0xFFFFFFFEu, 0u, (uint32_t)sweep, (uint32_t)(sweep >> 32), rp.k0, rp.k1, u);
+  uint32_t pk = 0x76543210u;
+#pragma unroll
+  for (int h = 0; h < 7; ++h) {
+    const int i = 7 - h;
+    const uint32_t half = (u[h >> 1] >> (16 * (h & 1))) & 0xFFFFu;
+    const int j = (int)((half * (uint32_t)(i + 1)) >> 16);
+    const uint32_t vi = (pk >> (4 * i)) & 0xFu, vj = (pk >> (4 * j)) & 0xFu;
+    pk &= ~((0xFu << (4 * i)) | (0xFu << (4 * j)));
+    pk |= (vi << (4 * j)) | (vj << (4 * i));
+  }
+  return (int)((pk >> (4 * phase)) & 0xFu);
+}
+
+__global__ void __launch_bounds__(256)
+k_rebin(const RepDev *__restrict__ reps, const int *__restrict__ prep, int P, int C,
+        const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+        int *__restrict__ cnt, int *__restrict__ spid, double *__restrict__ sx,
+        double *__restrict__ sy, double *__restrict__ sz, int cap,
+        const unsigned long long *__restrict__ sweep_base, int k, unsigned *__restrict__ err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned long long sweep = *sweep_base + (unsigned long long)k;
+  const int buf = (int)(sweep & 1ull);
+  if (i < C) cnt[(size_t)(buf ^ 1) * C + i] = 0;  // next sweep's buffer
+  if (i >= P) return;
+  const RepDev rp = reps[prep[i]];
+  double shx, shy, shz;
+  sweep_shift(rp, sweep, shx, shy, shz);
+  const double px = x[i], py = y[i], pz = z[i];
+  const int cx = shifted_cell(px, shx, rp.lx, rp.sx, rp.gx);
+  const int cy = shifted_cell(py, shy, rp.ly, rp.sy, rp.gy);
+  const int cz = shifted_cell(pz, shz, rp.lz, rp.sz, rp.gz);
+  const int c = rp.coff + (cx * rp.gy + cy) * rp.gz + cz;
+  const int slot = atomicAdd(&cnt[(size_t)buf * C + c], 1);
+  if (slot < cap) {
+    const size_t s = (size_t)c * cap + slot;
+    spid[s] = i - rp.poff;
+    sx[s] = px;
+    sy[s] = py;
+    sz[s] = pz;
+  } else {
+    atomicOr(err, kErrOverflow);
+  }
+}
+
+template <int CAP, int LPC>
+__global__ void __launch_bounds__(256)
+k_phase(const RepDev *__restrict__ reps, const int *__restrict__ aoff, int R, int A, int phase,
+        int k, const unsigned long long *__restrict__ sweep_base, double *__restrict__ x,
+        double *__restrict__ y, double *__restrict__ z, const int *__restrict__ cnt, int C,
+        const int *__restrict__ spid, double *__restrict__ sx, double *__restrict__ sy,
+        double *__restrict__ sz, unsigned long long *__restrict__ ctr) {
+  constexpr int LMAX = 27 * CAP;
+  extern __shared__ double smem[];
+  const int cpb = blockDim.x / LPC;
+  const int lane = threadIdx.x & (LPC - 1);
+  const int sub = threadIdx.x / LPC;
+  double *Lx = smem + (size_t)sub * (3 * LMAX);
+  double *Ly = Lx + LMAX;
+  double *Lz = Ly + LMAX;
+  int *Ls = reinterpret_cast<int *>(smem + (size_t)cpb * 3 * LMAX) + (size_t)sub * (2 * LMAX + CAP);
+  int *Lp = Ls + LMAX;
+  int *Lo = Lp + LMAX;
+
+  const int a = blockIdx.x * cpb + sub;
+  unsigned long long n_att = 0, n_acc = 0, n_left = 0;
+  int r = 0;
+  if (a < A) {
+    {
+      int lo = 0, hi = R;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (aoff[mid] <= a) lo = mid; else hi = mid;
+      }
+      r = lo;
+    }
+    const RepDev rp = reps[r];
+    const unsigned long long sweep = *sweep_base + (unsigned long long)k;
+    const int *cntb = cnt + (size_t)(sweep & 1ull) * C;
+    const int colour = sweep_colour(rp, sweep, phase);
+    double shx, shy, shz;
+    sweep_shift(rp, sweep, shx, shy, shz);
+
+    // active cell of this work item
+    const int hz2 = rp.gz >> 1, hy2 = rp.gy >> 1;
+    const int la = a - rp.aoff;
+    const int iz2 = la % hz2;
+    const int t2 = la / hz2;
+    const int iy2 = t2 % hy2;
+    const int ix2 = t2 / hy2;
+    const int cx = 2 * ix2 + (colour & 1);
+    const int cy = 2 * iy2 + ((colour >> 1) & 1);
+    const int cz = 2 * iz2 + ((colour >> 2) & 1);
+    const int cid = (cx * rp.gy + cy) * rp.gz + cz;
+
+    // lane j < 27 owns neighbour cell (j/9-1, (j/3)%3-1, j%3-1); 13 is self
+    int nj = 0, ncell = 0;
+    if (lane < 27) {
+      const int dx = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dz = lane % 3 - 1;
+      if (stencil_axis_ok(dx, rp.gx) && stencil_axis_ok(dy, rp.gy) && stencil_axis_ok(dz, rp.gz)) {
+        ncell = rp.coff + (wrap_index(cx + dx, rp.gx) * rp.gy + wrap_index(cy + dy, rp.gy)) * rp.gz +
+                wrap_index(cz + dz, rp.gz);
+        nj = cntb[ncell];
+        if (nj > CAP) nj = CAP;
+      }
+    }
+    int incl = nj;
+#pragma unroll
+    for (int d = 1; d < LPC; d <<= 1) {
+      const int v = __shfl_up(incl, d, LPC);
+      if (lane >= d) incl += v;
+    }
+    const int off = incl - nj;
+    const int T = __shfl(incl, LPC - 1, LPC);
+    const int nc = __shfl(nj, 13, LPC);
+    const int oc = __shfl(off, 13, LPC);
+
+    // stage the 27 cells' spheres in LDS (FP64 SoA)
+    for (int q = 0; q < nj; ++q) {
+      const int e = off + q;
+      const size_t s = (size_t)ncell * CAP + q;
+      Lx[e] = sx[s];
+      Ly[e] = sy[s];
+      Lz[e] = sz[s];
+      Ls[e] = (int)s;
+      Lp[e] = spid[s];
+    }
+    wave_lds_sync();
+    // occupants of the active cell in ascending particle id
+    if (lane < nc) {
+      const int me = Lp[oc + lane];
+      int rank = 0;
+      for (int m = 0; m < nc; ++m) rank += (Lp[oc + m] < me) ? 1 : 0;
+      Lo[rank] = oc + lane;
+    }
+    wave_lds_sync();
+
+    const uint32_t scale = nc > 0 ? 0xFFFFFFFFu / (uint32_t)nc : 1u;
+    for (int t = 0; t < nc; ++t) {
+      uint32_t w[4];
+      philox4x32_10((uint32_t)cid, (uint32_t)t, (uint32_t)sweep, (uint32_t)(sweep >> 32), rp.k0,
+                    rp.k1, w);
+      uint32_t kk = w[0] / scale;
+      for (uint32_t rr = 1; kk >= (uint32_t)nc; ++rr) {  // rng_get_int rejection
+        uint32_t w2[4];
+        philox4x32_10((uint32_t)cid, (uint32_t)t | (rr << 16), (uint32_t)sweep,
+                      (uint32_t)(sweep >> 32), rp.k0, rp.k1, w2);
+        kk = w2[0] / scale;
+      }
+      const int e = Lo[kk];
+      double nx = Lx[e] + (u32_unit_closed(w[1]) - 0.5) * rp.dr_max;
+      double ny = Ly[e] + (u32_unit_closed(w[2]) - 0.5) * rp.dr_max;
+      double nz = Lz[e] + (u32_unit_closed(w[3]) - 0.5) * rp.dr_max;
+      nx = wrap_once(nx, rp.lx);
+      ny = wrap_once(ny, rp.ly);
+      nz = wrap_once(nz, rp.lz);
+      ++n_att;
+      if (shifted_cell(nx, shx, rp.lx, rp.sx, rp.gx) != cx ||
+          shifted_cell(ny, shy, rp.ly, rp.sy, rp.gy) != cy ||
+          shifted_cell(nz, shz, rp.lz, rp.sz, rp.gz) != cz) {
+        ++n_left;
+        continue;
+      }
+      bool ov = false;
+      for (int q = lane; q < T; q += LPC) {
+        if (q != e &&
+            dist2(nx, ny, nz, Lx[q], Ly[q], Lz[q], rp.lx, rp.ly, rp.lz, rp.hx, rp.hy, rp.hz) < 1.0)
+          ov = true;
+      }
+      const unsigned long long bal = __ballot(ov);
+      const unsigned long long mine =
+          (LPC == 64) ? bal : ((bal >> (threadIdx.x & 32)) & 0xFFFFFFFFull);
+      if (mine == 0ull) {
+        if (lane == 0) {
+          Lx[e] = nx;
+          Ly[e] = ny;
+          Lz[e] = nz;
+        }
+        ++n_acc;
+      }
+      wave_lds_sync();
+    }
+    // write the active cell's spheres back (slot copy and by-id arrays)
+    if (lane < nc) {
+      const int e = oc + lane;
+      const int s = Ls[e];
+      const double px = Lx[e], py = Ly[e], pz = Lz[e];
+      sx[s] = px;
+      sy[s] = py;
+      sz[s] = pz;
+      const int g = rp.poff + Lp[e];
+      x[g] = px;
+      y[g] = py;
+      z[g] = pz;
+    }
+  }
+  if (lane == 0 && a < A) {
+    unsigned long long *c = ctr + (size_t)r * 3 * kSpread;
+    const int slot = blockIdx.x & (kSpread - 1);
+    if (n_att) atomicAdd(&c[slot], n_att);
+    if (n_acc) atomicAdd(&c[kSpread + slot], n_acc);
+    if (n_left) atomicAdd(&c[2 * kSpread + slot], n_left);
+  }
+}
+
+__global__ void k_advance(unsigned long long *sweep_base, int n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *sweep_base += (unsigned long long)n;
+}
+
+using PhaseFn = void (*)(const RepDev *, const int *, int, int, int, int,
+                         const unsigned long long *, double *, double *, double *, const int *,
+                         int, const int *, double *, double *, double *, unsigned long long *);
+
+PhaseFn phase_fn(int cap, int lpc) {
+  if (cap == 8) return lpc == 64 ? k_phase<8, 64> : k_phase<8, 32>;
+  return lpc == 64 ? k_phase<16, 64> : k_phase<16, 32>;
+}
+
+size_t phase_lds_bytes(int cap, int cpb) {
+  const size_t lmax = 27 * (size_t)cap;
+  return (size_t)cpb * (3 * lmax * sizeof(double) + (2 * lmax + cap) * sizeof(int));
+}
+
+bool tuning_ok(const hsmc_gpu_ctx *ctx, const hsmc_gpu_tuning &t) {
+  if (t.lanes_per_cell != 32 && t.lanes_per_cell != 64) return false;
+  if (t.cells_per_block < 1 || t.lanes_per_cell * t.cells_per_block > 256) return false;
+  if (phase_lds_bytes(ctx->cap, t.cells_per_block) > 64 * 1024) return false;
+  if (t.sweeps_per_graph < 0 || t.sweeps_per_graph > 512) return false;
+  return true;
+}
+
+// Enqueue the kernels of sweep k of the current batch on stream s.
+void enqueue_sweep(hsmc_gpu_ctx *ctx, int k, hipStream_t s, hipEvent_t *phase_events) {
+  const long work = std::max(ctx->Ptot, ctx->Ctot);
+  hipLaunchKernelGGL(k_rebin, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, ctx->d_reps,
+                     ctx->d_prep, (int)ctx->Ptot, (int)ctx->Ctot, ctx->d_x, ctx->d_y, ctx->d_z,
+                     ctx->d_cnt, ctx->d_spid, ctx->d_sx, ctx->d_sy, ctx->d_sz, ctx->cap,
+                     ctx->d_sweep, k, ctx->d_err);
+  const int lpc = ctx->tune.lanes_per_cell, cpb = ctx->tune.cells_per_block;
+  const unsigned blocks = (unsigned)((ctx->Atot + cpb - 1) / cpb);
+  const size_t lds = phase_lds_bytes(ctx->cap, cpb);
+  PhaseFn fn = phase_fn(ctx->cap, lpc);
+  for (int p = 0; p < 8; ++p) {
+    if (phase_events) (void)hipEventRecord(phase_events[2 * p], s);
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(lpc * cpb), lds, s, ctx->d_reps, ctx->d_aoff,
+                       ctx->R, (int)ctx->Atot, p, k, ctx->d_sweep, ctx->d_x, ctx->d_y, ctx->d_z,
+                       ctx->d_cnt, (int)ctx->Ctot, ctx->d_spid, ctx->d_sx, ctx->d_sy, ctx->d_sz,
+                       ctx->d_ctr);
+    if (phase_events) (void)hipEventRecord(phase_events[2 * p + 1], s);
+  }
+}
+
+int capture_graph(hsmc_gpu_ctx *ctx, int K) {
+  drop_graph(ctx);
+  HSMC_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < K; ++k) enqueue_sweep(ctx, k, ctx->stream, nullptr);
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, ctx->stream, ctx->d_sweep, K);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
+  ctx->graph = g;
+  e = hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    drop_graph(ctx);
+    return hip_fail(ctx, e, "hipGraphInstantiate");
+  }
+  ctx->graph_sweeps = K;
+  ctx->graph_tune = ctx->tune;
+  return HSMC_OK;
+}
+
+int run_sweeps(hsmc_gpu_ctx *ctx, int n) {
+  if (n <= 0) return HSMC_OK;
+  ctx->version++;
+  const int K = ctx->tune.sweeps_per_graph;
+  int done = 0;
+  if (K > 0 && n >= K) {
+    if (!ctx->gexec || ctx->graph_sweeps != K || ctx->graph_tune.lanes_per_cell != ctx->tune.lanes_per_cell ||
+        ctx->graph_tune.cells_per_block != ctx->tune.cells_per_block) {
+      int st = capture_graph(ctx, K);
+      if (st) return st;
+    }
+    for (; done + K <= n; done += K) HSMC_HIP(ctx, hipGraphLaunch(ctx->gexec, ctx->stream));
+  }
+  const int rem = n - done;
+  if (rem > 0) {
+    for (int k = 0; k < rem; ++k) enqueue_sweep(ctx, k, ctx->stream, nullptr);
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, ctx->stream, ctx->d_sweep, rem);
+    HSMC_HIP(ctx, hipGetLastError());
+  }
+  return HSMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hsmc_gpu_sweeps(hsmc_gpu_ctx *ctx, int n_sweeps) {
+  if (!ctx || n_sweeps < 0) return HSMC_ERR_BAD_ARG;
+  return run_sweeps(ctx, n_sweeps);
+}
+
+int hsmc_gpu_time_sweeps(hsmc_gpu_ctx *ctx, int n_sweeps, float *ms) {
+  if (!ctx || n_sweeps < 0 || !ms) return HSMC_ERR_BAD_ARG;
+  // capture outside the timed region
+  if (ctx->tune.sweeps_per_graph > 0 && n_sweeps >= ctx->tune.sweeps_per_graph &&
+      (!ctx->gexec || ctx->graph_sweeps != ctx->tune.sweeps_per_graph)) {
+    int st = capture_graph(ctx, ctx->tune.sweeps_per_graph);
+    if (st) return st;
+  }
+  HSMC_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  int st = run_sweeps(ctx, n_sweeps);
+  if (st) return st;
+  HSMC_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  HSMC_HIP(ctx, hipEventSynchronize(ctx->ev1));
+  HSMC_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+  return check_device_errors(ctx);
+}
+
+int hsmc_gpu_time_phase_kernel(hsmc_gpu_ctx *ctx, int n_sweeps, float *avg_ms, int32_t *launches) {
+  if (!ctx || n_sweeps <= 0 || !avg_ms) return HSMC_ERR_BAD_ARG;
+  hipEvent_t ev[16];
+  for (int i = 0; i < 16; ++i) HSMC_HIP(ctx, hipEventCreate(&ev[i]));
+  double total = 0.0;
+  int count = 0;
+  for (int s = 0; s < n_sweeps; ++s) {
+    enqueue_sweep(ctx, 0, ctx->stream, ev);
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, ctx->stream, ctx->d_sweep, 1);
+    HSMC_HIP(ctx, hipEventSynchronize(ev[15]));
+    for (int p = 0; p < 8; ++p) {
+      float ms = 0.f;
+      HSMC_HIP(ctx, hipEventElapsedTime(&ms, ev[2 * p], ev[2 * p + 1]));
+      total += ms;
+      ++count;
+    }
+  }
+  for (int i = 0; i < 16; ++i) (void)hipEventDestroy(ev[i]);
+  *avg_ms = (float)(total / count);
+  if (launches) *launches = count;
+  return check_device_errors(ctx);
+}
+
+int hsmc_gpu_counters(hsmc_gpu_ctx *ctx, int replica, uint64_t out[4]) {
+  if (!ctx || replica < 0 || replica >= ctx->R || !out) return HSMC_ERR_BAD_ARG;
+  std::vector<unsigned long long> c(3 * kSpread);
+  unsigned long long sw = 0;
+  HSMC_HIP(ctx, hipMemcpyAsync(c.data(), ctx->d_ctr + (size_t)replica * 3 * kSpread,
+                               sizeof(unsigned long long) * 3 * kSpread, hipMemcpyDeviceToHost,
+                               ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(&sw, ctx->d_sweep, sizeof(sw), hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int kind = 0; kind < 3; ++kind) {
+    unsigned long long s = 0;
+    for (int i = 0; i < kSpread; ++i) s += c[kind * kSpread + i];
+    out[kind] = s;
+  }
+  out[3] = sw;
+  return check_device_errors(ctx);
+}
+
+int hsmc_gpu_reset_counters(hsmc_gpu_ctx *ctx) {
+  if (!ctx) return HSMC_ERR_BAD_ARG;
+  HSMC_HIP(ctx, hipMemsetAsync(ctx->d_ctr, 0, sizeof(unsigned long long) * ctx->R * 3 * kSpread,
+                               ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HSMC_OK;
+}
+
+int hsmc_gpu_set_tuning(hsmc_gpu_ctx *ctx, const hsmc_gpu_tuning *t) {
+  if (!ctx || !t || !tuning_ok(ctx, *t)) return HSMC_ERR_BAD_ARG;
+  ctx->tune = *t;
+  drop_graph(ctx);
+  return HSMC_OK;
+}
+
+int hsmc_gpu_get_tuning(const hsmc_gpu_ctx *ctx, hsmc_gpu_tuning *t) {
+  if (!ctx || !t) return HSMC_ERR_BAD_ARG;
+  *t = ctx->tune;
+  return HSMC_OK;
+}
+
+int hsmc_gpu_autotune(hsmc_gpu_ctx *ctx, int sweeps_per_trial, hsmc_gpu_tuning *best) {
+  if (!ctx || sweeps_per_trial <= 0) return HSMC_ERR_BAD_ARG;
+  const size_t P = (size_t)ctx->Ptot;
+  // save positions, sweep counter and move counters; restore afterwards
+  double *save = (double *)scratch(ctx, 23, sizeof(double) * 3 * P);
+  if (!save) return HSMC_ERR_NO_MEMORY;
+  unsigned long long sweep0 = 0;
+  std::vector<unsigned long long> ctr0((size_t)ctx->R * 3 * kSpread);
+  HSMC_HIP(ctx, hipMemcpyAsync(save, ctx->d_x, sizeof(double) * P, hipMemcpyDeviceToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(save + P, ctx->d_y, sizeof(double) * P, hipMemcpyDeviceToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(save + 2 * P, ctx->d_z, sizeof(double) * P, hipMemcpyDeviceToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(&sweep0, ctx->d_sweep, sizeof(sweep0), hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(ctr0.data(), ctx->d_ctr, sizeof(unsigned long long) * ctr0.size(),
+                               hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const hsmc_gpu_tuning start = ctx->tune;
+  hsmc_gpu_tuning winner = start;
+  float best_ms = 1e30f;
+  const int lpcs[2] = {32, 64};
+  const int cpbs[4] = {1, 2, 4, 8};
+  const int graphs[3] = {0, 16, 64};
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 4; ++b)
+      for (int g = 0; g < 3; ++g) {
+        hsmc_gpu_tuning t{lpcs[a], cpbs[b], graphs[g], 0};
+        if (!tuning_ok(ctx, t)) continue;
+        if (t.sweeps_per_graph > 0 && t.sweeps_per_graph > sweeps_per_trial) continue;
+        ctx->tune = t;
+        drop_graph(ctx);
+        float ms = 0.f;
+        int st = hsmc_gpu_time_sweeps(ctx, std::max(2, sweeps_per_trial / 4), &ms);  // warm-up, capture
+        if (st == HSMC_OK) st = hsmc_gpu_time_sweeps(ctx, sweeps_per_trial, &ms);
+        if (st != HSMC_OK) {
+          ctx->tune = start;
+          return st;
+        }
+        if (ms < best_ms) {
+          best_ms = ms;
+          winner = t;
+        }
+      }
+  ctx->tune = winner;
+  drop_graph(ctx);
+  HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_x, save, sizeof(double) * P, hipMemcpyDeviceToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_y, save + P, sizeof(double) * P, hipMemcpyDeviceToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_z, save + 2 * P, sizeof(double) * P, hipMemcpyDeviceToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_sweep, &sweep0, sizeof(sweep0), hipMemcpyHostToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_ctr, ctr0.data(), sizeof(unsigned long long) * ctr0.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cnt, 0, sizeof(int) * 2 * ctx->cell_alloc, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (best) *best = winner;
+  return HSMC_OK;
+}
+
+}  // extern "C"

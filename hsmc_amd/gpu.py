@@ -1,0 +1,310 @@
+"""ctypes binding of libhsmc_gpu.so (the C ABI declared in include/hsmc_gpu.h).
+
+This is a thin host-side mirror for Python callers (tests, bench, analysis
+drivers).  There is no CPU fallback: every call goes to the HIP library, and
+loading fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libhsmc_gpu.so")
+
+HSMC_OK = 0
+ERRORS = {
+    -1: "CELL_OVERFLOW", -2: "CELL_TOO_SMALL", -3: "INIT_OVERLAP", -4: "HIP",
+    -5: "BAD_ARG", -7: "NO_MEMORY", -8: "NOT_SUPPORTED", -9: "NO_DEVICE",
+}
+
+EXPORTS = [
+    "hsmc_gpu_create", "hsmc_gpu_destroy", "hsmc_gpu_strerror", "hsmc_gpu_last_error",
+    "hsmc_gpu_sync", "hsmc_gpu_upload", "hsmc_gpu_download", "hsmc_gpu_set_box",
+    "hsmc_gpu_scale", "hsmc_gpu_grid", "hsmc_gpu_sweeps", "hsmc_gpu_time_sweeps",
+    "hsmc_gpu_counters", "hsmc_gpu_reset_counters", "hsmc_gpu_set_tuning",
+    "hsmc_gpu_get_tuning", "hsmc_gpu_autotune", "hsmc_gpu_time_phase_kernel",
+    "hsmc_gpu_cell_list", "hsmc_gpu_overlap_batch", "hsmc_gpu_overlap_scaled",
+    "hsmc_gpu_any_overlap_scaled", "hsmc_gpu_pair_dist", "hsmc_gpu_rdf_counts",
+    "hsmc_gpu_contact_counts", "hsmc_gpu_widom_points", "hsmc_gpu_widom_uniform",
+]
+
+
+class Replica(C.Structure):
+    _fields_ = [("n_part", C.c_int32), ("reserved", C.c_int32), ("lx", C.c_double),
+                ("ly", C.c_double), ("lz", C.c_double), ("dr_max", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("lanes_per_cell", C.c_int32), ("cells_per_block", C.c_int32),
+                ("sweeps_per_graph", C.c_int32), ("reserved", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return dict(lanes_per_cell=self.lanes_per_cell, cells_per_block=self.cells_per_block,
+                    sweeps_per_graph=self.sweeps_per_graph)
+
+
+class HsmcError(RuntimeError):
+    def __init__(self, code: int, where: str, detail: str = ""):
+        self.code = code
+        super().__init__(f"{where}: {ERRORS.get(code, code)} {detail}".strip())
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libhsmc_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                          "(make -C hsmc_amd/csrc); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, u64, f64 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64, C.c_double
+    P = C.POINTER
+    sig = {
+        "hsmc_gpu_create": (i32, [P(vp), i32, i32, P(Replica)]),
+        "hsmc_gpu_destroy": (i32, [vp]),
+        "hsmc_gpu_strerror": (C.c_char_p, [i32]),
+        "hsmc_gpu_last_error": (C.c_char_p, [vp]),
+        "hsmc_gpu_sync": (i32, [vp]),
+        "hsmc_gpu_upload": (i32, [vp, i32, P(f64)]),
+        "hsmc_gpu_download": (i32, [vp, i32, P(f64)]),
+        "hsmc_gpu_set_box": (i32, [vp, i32, f64, f64, f64]),
+        "hsmc_gpu_scale": (i32, [vp, i32, f64, f64, f64, f64]),
+        "hsmc_gpu_grid": (i32, [vp, i32, P(i32)]),
+        "hsmc_gpu_sweeps": (i32, [vp, i32]),
+        "hsmc_gpu_time_sweeps": (i32, [vp, i32, P(C.c_float)]),
+        "hsmc_gpu_counters": (i32, [vp, i32, P(u64)]),
+        "hsmc_gpu_reset_counters": (i32, [vp]),
+        "hsmc_gpu_set_tuning": (i32, [vp, P(Tuning)]),
+        "hsmc_gpu_get_tuning": (i32, [vp, P(Tuning)]),
+        "hsmc_gpu_autotune": (i32, [vp, i32, P(Tuning)]),
+        "hsmc_gpu_time_phase_kernel": (i32, [vp, i32, P(C.c_float), P(i32)]),
+        "hsmc_gpu_cell_list": (i32, [vp, i32, i32, i32, i32, i32, P(i32), P(i32)]),
+        "hsmc_gpu_overlap_batch": (i32, [vp, i32, P(i32), P(f64), i32, P(C.c_uint8)]),
+        "hsmc_gpu_overlap_scaled": (i32, [vp, i32, P(i32), P(f64), i32, P(C.c_uint8)]),
+        "hsmc_gpu_any_overlap_scaled": (i32, [vp, i32, P(f64), i32, P(C.c_uint8)]),
+        "hsmc_gpu_pair_dist": (i32, [vp, i32, P(i32), P(i32), i32, f64, P(f64)]),
+        "hsmc_gpu_rdf_counts": (i32, [vp, i32, f64, i32, f64, P(i64), P(i64)]),
+        "hsmc_gpu_contact_counts": (i32, [vp, i32, f64, i32, f64, P(i64), P(i64)]),
+        "hsmc_gpu_widom_points": (i32, [vp, i32, P(f64), i32, P(C.c_uint8), P(i64)]),
+        "hsmc_gpu_widom_uniform": (i32, [vp, i32, u64, u64, i64, i64, P(i64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a: np.ndarray, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+class Context:
+    """One device context holding one or more independent systems (replicas)."""
+
+    def __init__(self, replicas, device: int = 0):
+        L = load()
+        reps = (Replica * len(replicas))()
+        for k, r in enumerate(replicas):
+            reps[k].n_part = int(r["n"])
+            reps[k].lx, reps[k].ly, reps[k].lz = (float(v) for v in r["box"])
+            reps[k].dr_max = float(r.get("dr_max", 0.1))
+            reps[k].seed = int(r.get("seed", 1))
+        h = C.c_void_p()
+        st = L.hsmc_gpu_create(C.byref(h), device, len(replicas), reps)
+        if st != HSMC_OK:
+            raise HsmcError(st, "hsmc_gpu_create")
+        self.h = h
+        self.L = L
+        self.replicas = [dict(r) for r in replicas]
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.hsmc_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _ck(self, st: int, where: str) -> None:
+        if st != HSMC_OK:
+            msg = self.L.hsmc_gpu_last_error(self.h)
+            raise HsmcError(st, where, msg.decode() if msg else "")
+
+    def n(self, r: int = 0) -> int:
+        return int(self.replicas[r]["n"])
+
+    # --- configuration ---
+    def upload(self, conf4: np.ndarray, r: int = 0) -> None:
+        c = np.ascontiguousarray(conf4, dtype=np.float64)
+        assert c.shape == (self.n(r), 4)
+        self._ck(self.L.hsmc_gpu_upload(self.h, r, _p(c, C.c_double)), "upload")
+
+    def download(self, r: int = 0) -> np.ndarray:
+        out = np.zeros((self.n(r), 4))
+        self._ck(self.L.hsmc_gpu_download(self.h, r, _p(out, C.c_double)), "download")
+        return out
+
+    def set_box(self, box, r: int = 0) -> None:
+        self._ck(self.L.hsmc_gpu_set_box(self.h, r, *map(float, box)), "set_box")
+        self.replicas[r]["box"] = tuple(box)
+
+    def scale(self, sf: float, box, r: int = 0) -> None:
+        self._ck(self.L.hsmc_gpu_scale(self.h, r, float(sf), *map(float, box)), "scale")
+        self.replicas[r]["box"] = tuple(box)
+
+    def grid(self, r: int = 0) -> tuple:
+        g = np.zeros(3, dtype=np.int32)
+        self._ck(self.L.hsmc_gpu_grid(self.h, r, _p(g, C.c_int32)), "grid")
+        return tuple(int(v) for v in g)
+
+    # --- sweeps ---
+    def sweeps(self, n: int) -> None:
+        self._ck(self.L.hsmc_gpu_sweeps(self.h, int(n)), "sweeps")
+
+    def time_sweeps(self, n: int) -> float:
+        ms = C.c_float(0)
+        self._ck(self.L.hsmc_gpu_time_sweeps(self.h, int(n), C.byref(ms)), "time_sweeps")
+        return ms.value
+
+    def time_phase_kernel(self, n_sweeps: int) -> tuple:
+        ms = C.c_float(0)
+        cnt = C.c_int32(0)
+        self._ck(self.L.hsmc_gpu_time_phase_kernel(self.h, int(n_sweeps), C.byref(ms),
+                                                   C.byref(cnt)), "time_phase_kernel")
+        return ms.value, cnt.value
+
+    def sync(self) -> None:
+        self._ck(self.L.hsmc_gpu_sync(self.h), "sync")
+
+    def counters(self, r: int = 0) -> dict:
+        out = np.zeros(4, dtype=np.uint64)
+        self._ck(self.L.hsmc_gpu_counters(self.h, r, _p(out, C.c_uint64)), "counters")
+        return dict(attempts=int(out[0]), accepted=int(out[1]), left_cell=int(out[2]),
+                    sweeps=int(out[3]))
+
+    def reset_counters(self) -> None:
+        self._ck(self.L.hsmc_gpu_reset_counters(self.h), "reset_counters")
+
+    def set_tuning(self, lanes_per_cell: int, cells_per_block: int, sweeps_per_graph: int) -> None:
+        t = Tuning(lanes_per_cell, cells_per_block, sweeps_per_graph, 0)
+        self._ck(self.L.hsmc_gpu_set_tuning(self.h, C.byref(t)), "set_tuning")
+
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        self._ck(self.L.hsmc_gpu_get_tuning(self.h, C.byref(t)), "get_tuning")
+        return t.as_dict()
+
+    def autotune(self, sweeps_per_trial: int = 64) -> dict:
+        t = Tuning()
+        self._ck(self.L.hsmc_gpu_autotune(self.h, int(sweeps_per_trial), C.byref(t)), "autotune")
+        return t.as_dict()
+
+    # --- cell list and overlap tests ---
+    def cell_list(self, nxyz, max_part: int, r: int = 0) -> tuple:
+        nx, ny, nz = nxyz
+        C_ = nx * ny * nz
+        pc = np.zeros((C_, max_part), dtype=np.int32)
+        nm = np.zeros((C_, 27), dtype=np.int32)
+        self._ck(self.L.hsmc_gpu_cell_list(self.h, r, nx, ny, nz, max_part, _p(pc, C.c_int32),
+                                           _p(nm, C.c_int32)), "cell_list")
+        return pc, nm
+
+    def overlap_batch(self, idx, xyz, r: int = 0) -> np.ndarray:
+        i = np.ascontiguousarray(idx, dtype=np.int32)
+        q = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(len(i), dtype=np.uint8)
+        self._ck(self.L.hsmc_gpu_overlap_batch(self.h, r, _p(i, C.c_int32), _p(q, C.c_double),
+                                               len(i), _p(out, C.c_uint8)), "overlap_batch")
+        return out
+
+    def overlap_scaled(self, idx, sf, r: int = 0) -> np.ndarray:
+        i = np.ascontiguousarray(idx, dtype=np.int32)
+        s = np.ascontiguousarray(sf, dtype=np.float64)
+        out = np.zeros(len(i), dtype=np.uint8)
+        self._ck(self.L.hsmc_gpu_overlap_scaled(self.h, r, _p(i, C.c_int32), _p(s, C.c_double),
+                                                len(i), _p(out, C.c_uint8)), "overlap_scaled")
+        return out
+
+    def any_overlap_scaled(self, sf, r: int = 0) -> np.ndarray:
+        s = np.ascontiguousarray(sf, dtype=np.float64)
+        out = np.zeros(len(s), dtype=np.uint8)
+        self._ck(self.L.hsmc_gpu_any_overlap_scaled(self.h, r, _p(s, C.c_double), len(s),
+                                                    _p(out, C.c_uint8)), "any_overlap_scaled")
+        return out
+
+    def pair_dist(self, i, j, sf: float = 1.0, r: int = 0) -> np.ndarray:
+        a = np.ascontiguousarray(i, dtype=np.int32)
+        b = np.ascontiguousarray(j, dtype=np.int32)
+        out = np.zeros(len(a))
+        self._ck(self.L.hsmc_gpu_pair_dist(self.h, r, _p(a, C.c_int32), _p(b, C.c_int32), len(a),
+                                           float(sf), _p(out, C.c_double)), "pair_dist")
+        return out
+
+    # --- observables ---
+    def rdf_counts(self, dr_bin: float, nbins: int, rmax_eff: float, r: int = 0) -> tuple:
+        out = np.zeros(nbins, dtype=np.int64)
+        oor = C.c_int64(0)
+        self._ck(self.L.hsmc_gpu_rdf_counts(self.h, r, dr_bin, nbins, rmax_eff,
+                                            _p(out, C.c_int64), C.byref(oor)), "rdf_counts")
+        return out, oor.value
+
+    def contact_counts(self, dr_bin: float, nbins: int, rmax_eff: float, r: int = 0) -> tuple:
+        out = np.zeros(nbins, dtype=np.int64)
+        oor = C.c_int64(0)
+        self._ck(self.L.hsmc_gpu_contact_counts(self.h, r, dr_bin, nbins, rmax_eff,
+                                                _p(out, C.c_int64), C.byref(oor)), "contact_counts")
+        return out, oor.value
+
+    def widom_points(self, xyz, r: int = 0) -> np.ndarray:
+        q = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(len(q), dtype=np.uint8)
+        hits = C.c_int64(0)
+        self._ck(self.L.hsmc_gpu_widom_points(self.h, r, _p(q, C.c_double), len(q),
+                                              _p(out, C.c_uint8), C.byref(hits)), "widom_points")
+        return out
+
+    def widom_uniform(self, seed: int, sample: int, first: int, n: int, r: int = 0) -> int:
+        hits = C.c_int64(0)
+        self._ck(self.L.hsmc_gpu_widom_uniform(self.h, r, int(seed), int(sample), int(first),
+                                               int(n), C.byref(hits)), "widom_uniform")
+        return hits.value
+
+
+def lattice(type_: int, n: int, rho: float):
+    """FCC (type 2) or SC (type 1) starting lattice and box, as sim_box_init
+    and part_init build them (sim_info.c:32-71, :125-156)."""
+    import math
+    pc = 1 if type_ == 1 else 4
+    a = math.pow(pc / rho, 1.0 / 3.0)
+    L = n * a
+    pts = []
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                if type_ == 1:
+                    pts.append((i * a, j * a, k * a))
+                else:
+                    pts.append((i * a, j * a, k * a))
+                    pts.append(((i + 0.5) * a, (j + 0.5) * a, k * a))
+                    pts.append(((i + 0.5) * a, j * a, (k + 0.5) * a))
+                    pts.append((i * a, (j + 0.5) * a, (k + 0.5) * a))
+    conf = np.zeros((len(pts), 4))
+    conf[:, 0] = np.arange(len(pts))
+    conf[:, 1:] = np.array(pts)
+    return conf, (L, L, L)

@@ -1,0 +1,175 @@
+/*
+ * hsmc_gpu.h -- C ABI of the MI355X (gfx950) hard-sphere Monte Carlo device
+ * layer, libhsmc_gpu.so.
+ *
+ * The reference (fedluc/HSMC) has no plugin or FFI boundary: its hot path is
+ * a set of C functions over global state (moves.h, cell_list.h, nvt.h,
+ * npt.h, compute_*.h).  This header is the device-side boundary beneath the
+ * reference-named C host layer (include/hsmc.h); each entry point names the
+ * reference function whose work it performs.  Plain C types only: host
+ * buffers in and out, device state owned by the context.
+ *
+ * Conventions
+ *   - Every function returns an int status: HSMC_OK (0) or a negative code
+ *     below.  Codes that mirror one of the reference's exit(EXIT_FAILURE)
+ *     sites name that site.  hsmc_gpu_strerror() and hsmc_gpu_last_error()
+ *     give text.
+ *   - A context owns one HIP stream and the device memory of one or more
+ *     independent systems ("replicas").  Calls on one context are
+ *     stream-ordered and must come from one host thread at a time.
+ *   - Configurations cross the boundary in the reference's layout
+ *     (sim_info.h:20): N rows of {id, x, y, z} doubles.
+ *   - All distances are FP64 with the reference's evaluation order
+ *     (moves.c:400-431), so overlap decisions, cell lists, histogram counts
+ *     and Widom hits equal the reference's on the same configuration.
+ */
+#ifndef HSMC_GPU_H
+#define HSMC_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HSMC_OK                 0
+#define HSMC_ERR_CELL_OVERFLOW -1  /* more occupants than a cell holds (cell_list.c:229-232) */
+#define HSMC_ERR_CELL_TOO_SMALL -2 /* cell side below the sphere diameter (cell_list.c:126-129) */
+#define HSMC_ERR_INIT_OVERLAP  -3  /* overlapping starting lattice (sim_info.c:128,143) */
+#define HSMC_ERR_HIP           -4  /* a HIP runtime call failed */
+#define HSMC_ERR_BAD_ARG       -5  /* argument out of range */
+#define HSMC_ERR_NO_MEMORY     -7  /* device or host allocation failed */
+#define HSMC_ERR_NOT_SUPPORTED -8  /* request outside what the kernels handle */
+#define HSMC_ERR_NO_DEVICE     -9  /* no HIP device */
+
+typedef struct hsmc_gpu_ctx hsmc_gpu_ctx;
+
+/* One independent system.  lx/ly/lz are the box edges (sim_info.c:63-65),
+ * dr_max the full width of a trial displacement (moves.c:52-54), seed the
+ * 64-bit Philox4x32-10 key of its random stream. */
+typedef struct hsmc_gpu_replica {
+  int32_t  n_part;
+  int32_t  reserved;
+  double   lx, ly, lz;
+  double   dr_max;
+  uint64_t seed;
+} hsmc_gpu_replica;
+
+/* Launch shape of the sweep kernels (chosen by hsmc_gpu_autotune or set by
+ * hand).  lanes_per_cell: 32 or 64 lanes test one active cell;
+ * cells_per_block: active cells per workgroup; sweeps_per_graph: sweeps
+ * captured in one hipGraph launch (0 = direct launches). */
+typedef struct hsmc_gpu_tuning {
+  int32_t lanes_per_cell;
+  int32_t cells_per_block;
+  int32_t sweeps_per_graph;
+  int32_t reserved;
+} hsmc_gpu_tuning;
+
+/* ---- life cycle -------------------------------------------------------- */
+
+/* Create a context on HIP device `device` holding n_replicas systems.  The
+ * sweep grid of each replica is chosen here: the largest even number of
+ * cells per axis whose side exceeds one diameter. */
+int hsmc_gpu_create(hsmc_gpu_ctx **out, int device, int n_replicas,
+                    const hsmc_gpu_replica *replicas);
+int hsmc_gpu_destroy(hsmc_gpu_ctx *ctx);
+const char *hsmc_gpu_strerror(int code);
+const char *hsmc_gpu_last_error(const hsmc_gpu_ctx *ctx);
+int hsmc_gpu_sync(hsmc_gpu_ctx *ctx);
+
+/* ---- configuration (sim_info.c:179-190 part_config_get; io_config.c) ---- */
+
+/* Copy N x {id,x,y,z} into the replica (the reference's part_conf). */
+int hsmc_gpu_upload(hsmc_gpu_ctx *ctx, int replica, const double *conf4);
+/* Copy the replica's configuration out as N x {id,x,y,z}. */
+int hsmc_gpu_download(hsmc_gpu_ctx *ctx, int replica, double *conf4);
+/* Replace the box edges without moving particles. */
+int hsmc_gpu_set_box(hsmc_gpu_ctx *ctx, int replica, double lx, double ly, double lz);
+/* Accepted volume move (moves.c:129-142): multiply every coordinate by sf,
+ * install the new box, then wrap each particle as apply_pbc does
+ * (moves.c:215-226).  The sweep grid is re-chosen for the new box. */
+int hsmc_gpu_scale(hsmc_gpu_ctx *ctx, int replica, double sf,
+                   double lx, double ly, double lz);
+/* Sweep grid of the replica (cells per axis). */
+int hsmc_gpu_grid(const hsmc_gpu_ctx *ctx, int replica, int32_t out_g[3]);
+
+/* ---- sweeps (nvt.c:201-209 sweep_nvt; moves.c:27-80 part_move) ---------- */
+
+/* n_sweeps sweeps of every replica: per sweep a random shift of the cell
+ * grid and a random order of the 8 sublattices; each active cell makes as
+ * many displacement attempts as it has occupants, on uniformly chosen
+ * occupants; a move that leaves its cell is rejected.  N attempts per
+ * replica per sweep, like sweep_nvt. */
+int hsmc_gpu_sweeps(hsmc_gpu_ctx *ctx, int n_sweeps);
+/* Same, bracketed by HIP events on the context's stream; *ms = elapsed. */
+int hsmc_gpu_time_sweeps(hsmc_gpu_ctx *ctx, int n_sweeps, float *ms);
+/* Per-replica counters: attempts, accepted, rejected for leaving the cell,
+ * sweeps done (moves.c:22-24 part_moves / acc_part_moves, kept 64-bit). */
+int hsmc_gpu_counters(hsmc_gpu_ctx *ctx, int replica, uint64_t out[4]);
+int hsmc_gpu_reset_counters(hsmc_gpu_ctx *ctx);
+int hsmc_gpu_set_tuning(hsmc_gpu_ctx *ctx, const hsmc_gpu_tuning *t);
+int hsmc_gpu_get_tuning(const hsmc_gpu_ctx *ctx, hsmc_gpu_tuning *t);
+/* Time the candidate launch shapes on this context's own replicas (their
+ * state is saved and restored) and keep the fastest.  *best receives it. */
+int hsmc_gpu_autotune(hsmc_gpu_ctx *ctx, int sweeps_per_trial, hsmc_gpu_tuning *best);
+/* Average duration (ms) of one launch of the sweep's dominant kernel (the
+ * sublattice-phase kernel), measured with HIP events around each launch
+ * of n_sweeps sweeps on the context's stream. */
+int hsmc_gpu_time_phase_kernel(hsmc_gpu_ctx *ctx, int n_sweeps, float *avg_ms,
+                               int32_t *launches);
+
+/* ---- cell list (cell_list.c:142-175 cell_list_new, :253-301) ------------- */
+
+/* Fresh cell list on the reference's grid nx*ny*nz (size = L/n per axis):
+ * part_cell is C rows of max_part ints, [0] = count, then ascending particle
+ * ids, then -1 (the layout cell_list_new builds).  neigh_mat (C x 27, may be
+ * NULL) is the 27-cell stencil in the reference's order.  Cells are indexed
+ * ix*ny*nz + iy*nz + iz (equal to the reference's formula on cubic grids). */
+int hsmc_gpu_cell_list(hsmc_gpu_ctx *ctx, int replica, int nx, int ny, int nz,
+                       int max_part, int32_t *part_cell, int32_t *neigh_mat);
+
+/* ---- overlap tests (moves.c:157-212 check_overlap) ----------------------- */
+
+/* out[k] = 1 when particle idx[k], placed at xyz[3k..3k+2], overlaps any
+ * other particle (the test part_move makes, moves.c:60). */
+int hsmc_gpu_overlap_batch(hsmc_gpu_ctx *ctx, int replica, const int32_t *idx,
+                           const double *xyz, int m, uint8_t *out);
+/* out[k] = check_overlap(idx[k], sf[k], sf[k], sf[k]) with every coordinate
+ * and the box scaled by sf[k] (the test vol_move and presst make). */
+int hsmc_gpu_overlap_scaled(hsmc_gpu_ctx *ctx, int replica, const int32_t *idx,
+                            const double *sf, int m, uint8_t *out);
+/* no_overlap[k] = 1 when no pair overlaps after scaling by sf[k], k < n_sf
+ * (n_sf <= 64): vol_move's scan (moves.c:106-112) and presst_compute_hist
+ * (compute_press.c:243-273), as one all-particle reduction. */
+int hsmc_gpu_any_overlap_scaled(hsmc_gpu_ctx *ctx, int replica, const double *sf,
+                                int n_sf, uint8_t *no_overlap);
+/* compute_dist (moves.c:400-431) for pairs (i[k], j[k]) with scale sf. */
+int hsmc_gpu_pair_dist(hsmc_gpu_ctx *ctx, int replica, const int32_t *i,
+                       const int32_t *j, int m, double sf, double *out);
+
+/* ---- observables --------------------------------------------------------- */
+
+/* rdf_hist_compute (compute_rdf.c:110-128): number of pairs per bin
+ * (int)((dr-1)/dr_bin) with dr < rmax_eff (the reference's histogram is 2x
+ * these counts).  *out_of_range counts pairs whose bin falls outside
+ * [0, nbins). */
+int hsmc_gpu_rdf_counts(hsmc_gpu_ctx *ctx, int replica, double dr_bin, int nbins,
+                        double rmax_eff, int64_t *counts, int64_t *out_of_range);
+/* pressv_compute_hist (compute_press.c:123-165): contact pairs, same
+ * binning, rmax_eff = dr_bin * nbins + 1. */
+int hsmc_gpu_contact_counts(hsmc_gpu_ctx *ctx, int replica, double dr_bin, int nbins,
+                            double rmax_eff, int64_t *counts, int64_t *out_of_range);
+/* widom_check_overlap (compute_widom_chem_pot.c:82-120) for given trial
+ * points: hit[k] = 1 when the point is insertable; *hits = sum. */
+int hsmc_gpu_widom_points(hsmc_gpu_ctx *ctx, int replica, const double *xyz, int m,
+                          uint8_t *hit, int64_t *hits);
+/* widom_insertion (compute_widom_chem_pot.c:44-71) with n uniform points
+ * i = first..first+n-1 of Philox stream (seed, sample); *hits = insertable. */
+int hsmc_gpu_widom_uniform(hsmc_gpu_ctx *ctx, int replica, uint64_t seed,
+                           uint64_t sample, int64_t first, int64_t n, int64_t *hits);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

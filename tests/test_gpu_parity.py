@@ -1,0 +1,185 @@
+"""HIP kernels against the reference's golden vectors and the CPU oracle,
+through the C ABI.  Integer and decision outputs must be bit-exact; FP64
+positions after sweeps must be bit-exact against the oracle's restatement of
+the same Philox stream."""
+import math
+
+import numpy as np
+import pytest
+
+from hsmc_amd import gpu
+from oracle import oracle as O
+from tests.conftest import golden_cases
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["c1_lattice", "c1_mc", "c1_mc_dr1", "dense_sc", "n4000"]
+
+
+def _ctx_for(g, name, seed=1):
+    t, n, rho, ndr, mp = g[f"{name}/sys"]
+    o = O.System(int(t), int(n), rho=float(rho), neigh_dr=float(ndr), max_part=int(mp))
+    L = o.box()["lx"]
+    conf = g[f"{name}/conf"]
+    ctx = gpu.Context([dict(n=len(conf), box=(L, L, L), dr_max=0.1, seed=seed)])
+    ctx.upload(conf)
+    return ctx, o, conf, L, int(mp), float(ndr)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_cell_list_matches_reference(golden, name):
+    ctx, o, conf, L, mp, ndr = _ctx_for(golden, name)
+    n = int(math.floor(L / ndr))
+    pc, nm = ctx.cell_list((n, n, n), mp)
+    assert np.array_equal(pc, golden[f"{name}/ref_part_cell"])
+    assert np.array_equal(nm, golden[f"{name}/ref_neigh_mat"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_overlap_batch_matches_reference(golden, name):
+    ctx, *_ = _ctx_for(golden, name)
+    q = golden[f"{name}/overlap_q"]
+    out = ctx.overlap_batch(q[:, 0].astype(np.int32), q[:, 1:])
+    assert np.array_equal(out, golden[f"{name}/overlap_ref"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_overlap_scaled_matches_reference(golden, name):
+    ctx, o, conf, L, mp, ndr = _ctx_for(golden, name)
+    q = golden[f"{name}/scaled_q"]
+    ref = golden[f"{name}/scaled_ref"]
+    side = L / math.floor(L / ndr)
+    # the reference scans only its 27 cells; equal whenever they span 1/sf
+    keep = side >= 1.0 / q[:, 1]
+    out = ctx.overlap_scaled(q[keep, 0].astype(np.int32), q[keep, 1])
+    assert keep.sum() > 0
+    assert np.array_equal(out, ref[keep])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_pair_dist_matches_reference(golden, name):
+    ctx, *_ = _ctx_for(golden, name)
+    q = golden[f"{name}/dist_q"]
+    ref = golden[f"{name}/dist_ref"]
+    for sf in np.unique(q[:, 2]):
+        sel = q[:, 2] == sf
+        out = ctx.pair_dist(q[sel, 0].astype(np.int32), q[sel, 1].astype(np.int32), sf)
+        assert np.array_equal(out, ref[sel])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_rdf_counts_match_reference(golden, name):
+    ctx, *_ = _ctx_for(golden, name)
+    h = golden[f"{name}/rdf_hist_ref"]
+    rmax = float(golden[f"{name}/rdf_rmax_ref"][0])
+    cnt, oor = ctx.rdf_counts(0.01, len(h), rmax)
+    assert oor == 0
+    assert np.array_equal(2.0 * cnt, h)
+
+
+@pytest.mark.parametrize("name", [c for c in CASES if c != "c1_mc_dr1"])
+def test_contact_counts_match_reference(golden, name):
+    ctx, *_ = _ctx_for(golden, name)
+    h = golden[f"{name}/contact_hist_ref"]
+    rmax = float(golden[f"{name}/contact_rmax_ref"][0])
+    cnt, oor = ctx.contact_counts(0.01, len(h), rmax)
+    assert oor == 0
+    assert np.array_equal(2.0 * cnt, h)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_presst_flags_match_reference(golden, name):
+    ctx, o, conf, L, mp, ndr = _ctx_for(golden, name)
+    for tag in "ab":
+        dxi, xim = golden[f"{name}/presst_{tag}_args"]
+        nn = int(xim / dxi)
+        sf = np.array([math.pow(1 - (i + 1) * dxi, 1.0 / 3.0) for i in range(nn)])
+        side = L / math.floor(L / ndr)
+        if side < 1.0 / sf.min():
+            continue
+        out = ctx.any_overlap_scaled(sf)
+        assert np.array_equal(out.astype(float), golden[f"{name}/presst_{tag}_ref"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_widom_points_match_reference(golden, name):
+    ctx, *_ = _ctx_for(golden, name)
+    out = ctx.widom_points(golden[f"{name}/widom_q"])
+    assert np.array_equal(out, golden[f"{name}/widom_ref"])
+
+
+def test_widom_uniform_matches_oracle(golden):
+    ctx, o, conf, L, mp, ndr = _ctx_for(golden, "c1_mc")
+    o.set_config(conf)
+    for sample, first, n in ((0, 0, 5000), (3, 1000, 7000), (2**33 + 5, 123456789, 4000)):
+        assert ctx.widom_uniform(77, sample, first, n) == o.widom_philox(77, sample, first, n)
+
+
+def test_sqrt_and_division_correctly_rounded():
+    # pair distances against numpy (IEEE sqrt) on 2e5 random pairs
+    rng = np.random.default_rng(3)
+    n = 4000
+    L = 20.0
+    conf = np.zeros((n, 4))
+    conf[:, 0] = np.arange(n)
+    conf[:, 1:] = rng.random((n, 3)) * L
+    ctx = gpu.Context([dict(n=n, box=(L, L, L))])
+    ctx.upload(conf)
+    i = rng.integers(0, n, 200000).astype(np.int32)
+    j = rng.integers(0, n, 200000).astype(np.int32)
+    out = ctx.pair_dist(i, j)
+    d = conf[i, 1:] - conf[j, 1:]
+    d = np.where(d > L / 2, d - L, np.where(d < -L / 2, d + L, d))
+    ref = np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2])
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("n,rho,sweeps,seed", [(5, 0.5, 6, 3), (10, 0.85, 4, 9), (4, 0.93, 5, 21)])
+def test_sweeps_match_oracle_restatement(n, rho, sweeps, seed):
+    o = O.System(2, n, rho=rho, neigh_dr=1.0, max_part=10, seed=1)
+    o.sweep_nvt(3)                       # a disordered start
+    conf = o.config()
+    L = o.box()["lx"]
+    ctx = gpu.Context([dict(n=len(conf), box=(L, L, L), dr_max=0.1, seed=seed)])
+    ctx.upload(conf)
+    g = ctx.grid()
+    cb = O.Checkerboard(conf, (L, L, L), g, 0.1, seed)
+    ctx.sweeps(sweeps)
+    cb.sweeps(sweeps)
+    out = ctx.download()
+    assert np.array_equal(out, cb.config())
+    c = ctx.counters()
+    ref = cb.counters()
+    assert (c["attempts"], c["accepted"], c["left_cell"], c["sweeps"]) == tuple(int(v) for v in ref)
+
+
+def test_sweeps_graph_and_direct_agree():
+    o = O.System(2, 8, rho=0.8, neigh_dr=1.0, max_part=10)
+    conf = o.config()
+    L = o.box()["lx"]
+    outs = []
+    for spg in (0, 4):
+        ctx = gpu.Context([dict(n=len(conf), box=(L, L, L), dr_max=0.12, seed=5)])
+        ctx.set_tuning(32, 8, spg)
+        ctx.upload(conf)
+        ctx.sweeps(10)
+        outs.append(ctx.download())
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_full_size_sweeps_keep_hard_core():
+    # N = 32000, rho = 0.85 (fcc 4*20^3): 50 sweeps, no overlap, all in box
+    conf, box = gpu.lattice(2, 20, 0.85)
+    ctx = gpu.Context([dict(n=len(conf), box=box, dr_max=0.1, seed=2)])
+    ctx.upload(conf)
+    ctx.sweeps(50)
+    c = ctx.counters()
+    assert c["attempts"] == 50 * 32000 and c["sweeps"] == 50
+    assert 0.3 < c["accepted"] / c["attempts"] < 0.95
+    out = ctx.download()
+    L = box[0]
+    assert ((out[:, 1:] >= 0) & (out[:, 1:] <= L)).all()
+    assert ctx.any_overlap_scaled([1.0])[0] == 1
+    # contact pairs only at r >= 1: the histogram's bin of r < 1 is empty
+    cnt, oor = ctx.contact_counts(0.01, 5, 1.05)
+    assert oor == 0 and cnt.sum() > 0
