@@ -11,8 +11,10 @@
 //   reps             RepDev[R]   box, grid and RNG key of each replica
 //   aoff             int32[R+1]  prefix of active cells per phase
 //   sweep            uint64      sweeps done (read by every sweep kernel)
-//   ctr              uint64[R*3*16] attempts / accepted / left-cell, spread
-//                                over 16 slots to limit atomic contention
+//   cacc, cleft      uint32[C]   accepted moves / moves rejected for leaving
+//                                the cell, per sweep cell; each cell is
+//                                updated by the one work item that owns it in
+//                                its phase, so plain stores suffice
 //   err              uint32      sticky error bits set by kernels
 #pragma once
 #include <hip/hip_runtime.h>
@@ -24,7 +26,6 @@
 
 namespace hsmc {
 
-constexpr int kSpread = 16;          // counter slots per replica and kind
 constexpr unsigned kErrOverflow = 1u;  // a cell received more than cap occupants
 constexpr unsigned kErrListFull = 2u;  // an LDS candidate list overflowed
 
@@ -62,7 +63,11 @@ struct hsmc_gpu_ctx {
   int *d_spid = nullptr;
   double *d_sx = nullptr, *d_sy = nullptr, *d_sz = nullptr;
   unsigned long long *d_sweep = nullptr;
-  unsigned long long *d_ctr = nullptr;
+  unsigned int *d_cacc = nullptr, *d_cleft = nullptr;
+  // host totals per replica: accepted, left-cell (folded from the per-cell
+  // device counters), and the sweep counter value at the last reset
+  std::vector<unsigned long long> tot_acc, tot_left;
+  unsigned long long sweep_at_reset = 0;
   unsigned int *d_err = nullptr;
   hsmc_gpu_tuning tune{32, 8, 32, 0};
   hipGraph_t graph = nullptr;
@@ -98,6 +103,8 @@ int sweep_grid_cells(double L);
 int check_device_errors(hsmc_gpu_ctx *ctx);
 // Invalidate any captured sweep graph.
 void drop_graph(hsmc_gpu_ctx *ctx);
+// Add the per-cell device counters into the host totals and clear them.
+int fold_counters(hsmc_gpu_ctx *ctx);
 
 }  // namespace hsmc
 

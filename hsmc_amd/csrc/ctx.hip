@@ -55,7 +55,35 @@ void drop_graph(hsmc_gpu_ctx *ctx) {
   ctx->graph_sweeps = 0;
 }
 
+int fold_counters(hsmc_gpu_ctx *ctx) {
+  if (!ctx->d_cacc || ctx->Ctot == 0) return HSMC_OK;
+  std::vector<unsigned int> a(ctx->Ctot), l(ctx->Ctot);
+  HSMC_HIP(ctx, hipMemcpyAsync(a.data(), ctx->d_cacc, sizeof(unsigned int) * ctx->Ctot,
+                               hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(l.data(), ctx->d_cleft, sizeof(unsigned int) * ctx->Ctot,
+                               hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cacc, 0, sizeof(unsigned int) * ctx->Ctot, ctx->stream));
+  HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cleft, 0, sizeof(unsigned int) * ctx->Ctot, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int r = 0; r < ctx->R; ++r) {
+    const RepHost &h = ctx->reps[r];
+    const long n = (long)h.gx * h.gy * h.gz;
+    unsigned long long sa = 0, sl = 0;
+    for (long c = h.coff; c < h.coff + n; ++c) {
+      sa += a[c];
+      sl += l[c];
+    }
+    ctx->tot_acc[r] += sa;
+    ctx->tot_left[r] += sl;
+  }
+  return HSMC_OK;
+}
+
 int layout(hsmc_gpu_ctx *ctx) {
+  {
+    int st = fold_counters(ctx);  // the cell arrays may move
+    if (st) return st;
+  }
   long coff = 0, aoff = 0;
   bool big = false;
   std::vector<RepDev> rd(ctx->R);
@@ -99,14 +127,18 @@ int layout(hsmc_gpu_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(ctx->d_cnt); (void)hipFree(ctx->d_spid);
     (void)hipFree(ctx->d_sx); (void)hipFree(ctx->d_sy); (void)hipFree(ctx->d_sz);
+    (void)hipFree(ctx->d_cacc); (void)hipFree(ctx->d_cleft);
     ctx->d_cnt = nullptr; ctx->d_spid = nullptr;
     ctx->d_sx = ctx->d_sy = ctx->d_sz = nullptr;
+    ctx->d_cacc = ctx->d_cleft = nullptr;
     size_t alloc = need + need / 8 + 64;
     HSMC_HIP(ctx, hipMalloc(&ctx->d_cnt, sizeof(int) * 2 * alloc));
     HSMC_HIP(ctx, hipMalloc(&ctx->d_spid, sizeof(int) * alloc * cap));
     HSMC_HIP(ctx, hipMalloc(&ctx->d_sx, sizeof(double) * alloc * cap));
     HSMC_HIP(ctx, hipMalloc(&ctx->d_sy, sizeof(double) * alloc * cap));
     HSMC_HIP(ctx, hipMalloc(&ctx->d_sz, sizeof(double) * alloc * cap));
+    HSMC_HIP(ctx, hipMalloc(&ctx->d_cacc, sizeof(unsigned int) * alloc));
+    HSMC_HIP(ctx, hipMalloc(&ctx->d_cleft, sizeof(unsigned int) * alloc));
     ctx->cell_alloc = alloc;
     ctx->cap = cap;
   }
@@ -120,6 +152,8 @@ int layout(hsmc_gpu_ctx *ctx) {
       ctx->tune.cells_per_block >>= 1;
   }
   HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cnt, 0, sizeof(int) * 2 * ctx->cell_alloc, ctx->stream));
+  HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cacc, 0, sizeof(unsigned int) * ctx->cell_alloc, ctx->stream));
+  HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cleft, 0, sizeof(unsigned int) * ctx->cell_alloc, ctx->stream));
   HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_reps, rd.data(), sizeof(RepDev) * ctx->R,
                                hipMemcpyHostToDevice, ctx->stream));
   HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_aoff, ao.data(), sizeof(int) * (ctx->R + 1),
@@ -212,6 +246,8 @@ int hsmc_gpu_create(hsmc_gpu_ctx **out, int device, int n_replicas,
     poff += replicas[r].n_part;
   }
   ctx->Ptot = ptot;
+  ctx->tot_acc.assign(n_replicas, 0);
+  ctx->tot_left.assign(n_replicas, 0);
   int st = HSMC_OK;
   auto fail = [&](int code) {
     hsmc_gpu_destroy(ctx);
@@ -229,12 +265,10 @@ int hsmc_gpu_create(hsmc_gpu_ctx **out, int device, int n_replicas,
       hipMalloc(&ctx->d_y, sizeof(double) * ptot) != hipSuccess ||
       hipMalloc(&ctx->d_z, sizeof(double) * ptot) != hipSuccess ||
       hipMalloc(&ctx->d_sweep, sizeof(unsigned long long)) != hipSuccess ||
-      hipMalloc(&ctx->d_ctr, sizeof(unsigned long long) * n_replicas * 3 * kSpread) != hipSuccess ||
       hipMalloc(&ctx->d_err, sizeof(unsigned int)) != hipSuccess)
     return fail(HSMC_ERR_NO_MEMORY);
   if (hipMemcpy(ctx->d_prep, prep.data(), sizeof(int) * ptot, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(ctx->d_sweep, 0, sizeof(unsigned long long)) != hipSuccess ||
-      hipMemset(ctx->d_ctr, 0, sizeof(unsigned long long) * n_replicas * 3 * kSpread) != hipSuccess ||
       hipMemset(ctx->d_err, 0, sizeof(unsigned int)) != hipSuccess ||
       hipMemset(ctx->d_x, 0, sizeof(double) * ptot) != hipSuccess ||
       hipMemset(ctx->d_y, 0, sizeof(double) * ptot) != hipSuccess ||
@@ -252,7 +286,7 @@ int hsmc_gpu_destroy(hsmc_gpu_ctx *ctx) {
   drop_graph(ctx);
   void *bufs[] = {ctx->d_reps, ctx->d_aoff, ctx->d_prep, ctx->d_x, ctx->d_y, ctx->d_z,
                   ctx->d_cnt, ctx->d_spid, ctx->d_sx, ctx->d_sy, ctx->d_sz, ctx->d_sweep,
-                  ctx->d_ctr, ctx->d_err};
+                  ctx->d_cacc, ctx->d_cleft, ctx->d_err};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (int i = 0; i < hsmc_gpu_ctx::kScratch; ++i)

@@ -101,7 +101,8 @@ k_phase(const RepDev *__restrict__ reps, const int *__restrict__ aoff, int R, in
         int k, const unsigned long long *__restrict__ sweep_base, double *__restrict__ x,
         double *__restrict__ y, double *__restrict__ z, const int *__restrict__ cnt, int C,
         const int *__restrict__ spid, double *__restrict__ sx, double *__restrict__ sy,
-        double *__restrict__ sz, unsigned long long *__restrict__ ctr) {
+        double *__restrict__ sz, unsigned int *__restrict__ cacc,
+        unsigned int *__restrict__ cleft) {
   constexpr int LMAX = 27 * CAP;
   extern __shared__ double smem[];
   const int cpb = blockDim.x / LPC;
@@ -115,143 +116,155 @@ k_phase(const RepDev *__restrict__ reps, const int *__restrict__ aoff, int R, in
   int *Lo = Lp + LMAX;
 
   const int a = blockIdx.x * cpb + sub;
-  unsigned long long n_att = 0, n_acc = 0, n_left = 0;
+  if (a >= A) return;  // the whole lane group leaves together
   int r = 0;
-  if (a < A) {
-    {
-      int lo = 0, hi = R;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (aoff[mid] <= a) lo = mid; else hi = mid;
-      }
-      r = lo;
+  {
+    int lo = 0, hi = R;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (aoff[mid] <= a) lo = mid; else hi = mid;
     }
-    const RepDev rp = reps[r];
-    const unsigned long long sweep = *sweep_base + (unsigned long long)k;
-    const int *cntb = cnt + (size_t)(sweep & 1ull) * C;
-    const int colour = sweep_colour(rp, sweep, phase);
-    double shx, shy, shz;
-    sweep_shift(rp, sweep, shx, shy, shz);
+    r = lo;
+  }
+  const RepDev rp = reps[r];
+  const unsigned long long sweep = *sweep_base + (unsigned long long)k;
+  const int *cntb = cnt + (size_t)(sweep & 1ull) * C;
+  const int colour = sweep_colour(rp, sweep, phase);
+  double shx, shy, shz;
+  sweep_shift(rp, sweep, shx, shy, shz);
 
-    // active cell of this work item
-    const int hz2 = rp.gz >> 1, hy2 = rp.gy >> 1;
-    const int la = a - rp.aoff;
-    const int iz2 = la % hz2;
-    const int t2 = la / hz2;
-    const int iy2 = t2 % hy2;
-    const int ix2 = t2 / hy2;
-    const int cx = 2 * ix2 + (colour & 1);
-    const int cy = 2 * iy2 + ((colour >> 1) & 1);
-    const int cz = 2 * iz2 + ((colour >> 2) & 1);
-    const int cid = (cx * rp.gy + cy) * rp.gz + cz;
+  // active cell of this work item
+  const int hz2 = rp.gz >> 1, hy2 = rp.gy >> 1;
+  const int la = a - rp.aoff;
+  const int iz2 = la % hz2;
+  const int t2 = la / hz2;
+  const int iy2 = t2 % hy2;
+  const int ix2 = t2 / hy2;
+  const int cx = 2 * ix2 + (colour & 1);
+  const int cy = 2 * iy2 + ((colour >> 1) & 1);
+  const int cz = 2 * iz2 + ((colour >> 2) & 1);
+  const int cid = (cx * rp.gy + cy) * rp.gz + cz;
 
-    // lane j < 27 owns neighbour cell (j/9-1, (j/3)%3-1, j%3-1); 13 is self
-    int nj = 0, ncell = 0;
-    if (lane < 27) {
-      const int dx = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dz = lane % 3 - 1;
-      if (stencil_axis_ok(dx, rp.gx) && stencil_axis_ok(dy, rp.gy) && stencil_axis_ok(dz, rp.gz)) {
-        ncell = rp.coff + (wrap_index(cx + dx, rp.gx) * rp.gy + wrap_index(cy + dy, rp.gy)) * rp.gz +
-                wrap_index(cz + dz, rp.gz);
-        nj = cntb[ncell];
-        if (nj > CAP) nj = CAP;
-      }
-    }
-    int incl = nj;
-#pragma unroll
-    for (int d = 1; d < LPC; d <<= 1) {
-      const int v = __shfl_up(incl, d, LPC);
-      if (lane >= d) incl += v;
-    }
-    const int off = incl - nj;
-    const int T = __shfl(incl, LPC - 1, LPC);
-    const int nc = __shfl(nj, 13, LPC);
-    const int oc = __shfl(off, 13, LPC);
-
-    // stage the 27 cells' spheres in LDS (FP64 SoA)
-    for (int q = 0; q < nj; ++q) {
-      const int e = off + q;
-      const size_t s = (size_t)ncell * CAP + q;
-      Lx[e] = sx[s];
-      Ly[e] = sy[s];
-      Lz[e] = sz[s];
-      Ls[e] = (int)s;
-      Lp[e] = spid[s];
-    }
-    wave_lds_sync();
-    // occupants of the active cell in ascending particle id
-    if (lane < nc) {
-      const int me = Lp[oc + lane];
-      int rank = 0;
-      for (int m = 0; m < nc; ++m) rank += (Lp[oc + m] < me) ? 1 : 0;
-      Lo[rank] = oc + lane;
-    }
-    wave_lds_sync();
-
-    const uint32_t scale = nc > 0 ? 0xFFFFFFFFu / (uint32_t)nc : 1u;
-    for (int t = 0; t < nc; ++t) {
-      uint32_t w[4];
-      philox4x32_10((uint32_t)cid, (uint32_t)t, (uint32_t)sweep, (uint32_t)(sweep >> 32), rp.k0,
-                    rp.k1, w);
-      uint32_t kk = w[0] / scale;
-      for (uint32_t rr = 1; kk >= (uint32_t)nc; ++rr) {  // rng_get_int rejection
-        uint32_t w2[4];
-        philox4x32_10((uint32_t)cid, (uint32_t)t | (rr << 16), (uint32_t)sweep,
-                      (uint32_t)(sweep >> 32), rp.k0, rp.k1, w2);
-        kk = w2[0] / scale;
-      }
-      const int e = Lo[kk];
-      double nx = Lx[e] + (u32_unit_closed(w[1]) - 0.5) * rp.dr_max;
-      double ny = Ly[e] + (u32_unit_closed(w[2]) - 0.5) * rp.dr_max;
-      double nz = Lz[e] + (u32_unit_closed(w[3]) - 0.5) * rp.dr_max;
-      nx = wrap_once(nx, rp.lx);
-      ny = wrap_once(ny, rp.ly);
-      nz = wrap_once(nz, rp.lz);
-      ++n_att;
-      if (shifted_cell(nx, shx, rp.lx, rp.sx, rp.gx) != cx ||
-          shifted_cell(ny, shy, rp.ly, rp.sy, rp.gy) != cy ||
-          shifted_cell(nz, shz, rp.lz, rp.sz, rp.gz) != cz) {
-        ++n_left;
-        continue;
-      }
-      bool ov = false;
-      for (int q = lane; q < T; q += LPC) {
-        if (q != e &&
-            dist2(nx, ny, nz, Lx[q], Ly[q], Lz[q], rp.lx, rp.ly, rp.lz, rp.hx, rp.hy, rp.hz) < 1.0)
-          ov = true;
-      }
-      const unsigned long long bal = __ballot(ov);
-      const unsigned long long mine =
-          (LPC == 64) ? bal : ((bal >> (threadIdx.x & 32)) & 0xFFFFFFFFull);
-      if (mine == 0ull) {
-        if (lane == 0) {
-          Lx[e] = nx;
-          Ly[e] = ny;
-          Lz[e] = nz;
-        }
-        ++n_acc;
-      }
-      wave_lds_sync();
-    }
-    // write the active cell's spheres back (slot copy and by-id arrays)
-    if (lane < nc) {
-      const int e = oc + lane;
-      const int s = Ls[e];
-      const double px = Lx[e], py = Ly[e], pz = Lz[e];
-      sx[s] = px;
-      sy[s] = py;
-      sz[s] = pz;
-      const int g = rp.poff + Lp[e];
-      x[g] = px;
-      y[g] = py;
-      z[g] = pz;
+  // lane j < 27 owns neighbour cell (j/9-1, (j/3)%3-1, j%3-1); 13 is self
+  int nj = 0, ncell = 0;
+  if (lane < 27) {
+    const int dx = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dz = lane % 3 - 1;
+    if (stencil_axis_ok(dx, rp.gx) && stencil_axis_ok(dy, rp.gy) && stencil_axis_ok(dz, rp.gz)) {
+      ncell = rp.coff + (wrap_index(cx + dx, rp.gx) * rp.gy + wrap_index(cy + dy, rp.gy)) * rp.gz +
+              wrap_index(cz + dz, rp.gz);
+      nj = cntb[ncell];
+      if (nj > CAP) nj = CAP;
     }
   }
-  if (lane == 0 && a < A) {
-    unsigned long long *c = ctr + (size_t)r * 3 * kSpread;
-    const int slot = blockIdx.x & (kSpread - 1);
-    if (n_att) atomicAdd(&c[slot], n_att);
-    if (n_acc) atomicAdd(&c[kSpread + slot], n_acc);
-    if (n_left) atomicAdd(&c[2 * kSpread + slot], n_left);
+  int incl = nj;
+#pragma unroll
+  for (int d = 1; d < LPC; d <<= 1) {
+    const int v = __shfl_up(incl, d, LPC);
+    if (lane >= d) incl += v;
+  }
+  const int off = incl - nj;
+  const int T = __shfl(incl, LPC - 1, LPC);
+  const int nc = __shfl(nj, 13, LPC);
+  const int oc = __shfl(off, 13, LPC);
+
+  // lane t < nc draws attempt t: occupant rank (rng_get_int rule) and the
+  // displacement (r - 0.5) * dr_max per axis, off the serial path below
+  uint32_t my_k = 0;
+  double my_dx = 0.0, my_dy = 0.0, my_dz = 0.0;
+  if (lane < nc) {
+    const uint32_t scale = 0xFFFFFFFFu / (uint32_t)nc;
+    uint32_t w[4];
+    philox4x32_10((uint32_t)cid, (uint32_t)lane, (uint32_t)sweep, (uint32_t)(sweep >> 32), rp.k0,
+                  rp.k1, w);
+    uint32_t kk = w[0] / scale;
+    for (uint32_t rr = 1; kk >= (uint32_t)nc; ++rr) {
+      uint32_t w2[4];
+      philox4x32_10((uint32_t)cid, (uint32_t)lane | (rr << 16), (uint32_t)sweep,
+                    (uint32_t)(sweep >> 32), rp.k0, rp.k1, w2);
+      kk = w2[0] / scale;
+    }
+    my_k = kk;
+    my_dx = (u32_unit_closed(w[1]) - 0.5) * rp.dr_max;
+    my_dy = (u32_unit_closed(w[2]) - 0.5) * rp.dr_max;
+    my_dz = (u32_unit_closed(w[3]) - 0.5) * rp.dr_max;
+  }
+
+  // stage the 27 cells' spheres in LDS (FP64 SoA)
+  for (int q = 0; q < nj; ++q) {
+    const int e = off + q;
+    const size_t s = (size_t)ncell * CAP + q;
+    Lx[e] = sx[s];
+    Ly[e] = sy[s];
+    Lz[e] = sz[s];
+    Ls[e] = (int)s;
+    Lp[e] = spid[s];
+  }
+  wave_lds_sync();
+  // occupants of the active cell in ascending particle id
+  if (lane < nc) {
+    const int me = Lp[oc + lane];
+    int rank = 0;
+    for (int m = 0; m < nc; ++m) rank += (Lp[oc + m] < me) ? 1 : 0;
+    Lo[rank] = oc + lane;
+  }
+  wave_lds_sync();
+
+  unsigned int n_acc = 0, n_left = 0;
+  for (int t = 0; t < nc; ++t) {
+    const int kk = __shfl((int)my_k, t, LPC);
+    const double ddx = __shfl(my_dx, t, LPC);
+    const double ddy = __shfl(my_dy, t, LPC);
+    const double ddz = __shfl(my_dz, t, LPC);
+    const int e = Lo[kk];
+    double nx = Lx[e] + ddx;
+    double ny = Ly[e] + ddy;
+    double nz = Lz[e] + ddz;
+    nx = wrap_once(nx, rp.lx);
+    ny = wrap_once(ny, rp.ly);
+    nz = wrap_once(nz, rp.lz);
+    if (shifted_cell(nx, shx, rp.lx, rp.sx, rp.gx) != cx ||
+        shifted_cell(ny, shy, rp.ly, rp.sy, rp.gy) != cy ||
+        shifted_cell(nz, shz, rp.lz, rp.sz, rp.gz) != cz) {
+      ++n_left;
+      continue;
+    }
+    bool ov = false;
+    for (int q = lane; q < T; q += LPC) {
+      if (q != e &&
+          dist2(nx, ny, nz, Lx[q], Ly[q], Lz[q], rp.lx, rp.ly, rp.lz, rp.hx, rp.hy, rp.hz) < 1.0)
+        ov = true;
+    }
+    const unsigned long long bal = __ballot(ov);
+    const unsigned long long mine =
+        (LPC == 64) ? bal : ((bal >> (threadIdx.x & 32)) & 0xFFFFFFFFull);
+    if (mine == 0ull) {
+      if (lane == 0) {
+        Lx[e] = nx;
+        Ly[e] = ny;
+        Lz[e] = nz;
+      }
+      ++n_acc;
+    }
+    wave_lds_sync();
+  }
+  // write the active cell's spheres back (slot copy and by-id arrays)
+  if (lane < nc) {
+    const int e = oc + lane;
+    const int s = Ls[e];
+    const double px = Lx[e], py = Ly[e], pz = Lz[e];
+    sx[s] = px;
+    sy[s] = py;
+    sz[s] = pz;
+    const int g = rp.poff + Lp[e];
+    x[g] = px;
+    y[g] = py;
+    z[g] = pz;
+  }
+  // per-cell move counters: this work item is the cell's only writer
+  if (lane == 13 && (n_acc | n_left)) {
+    const int c = rp.coff + cid;
+    if (n_acc) cacc[c] += n_acc;
+    if (n_left) cleft[c] += n_left;
   }
 }
 
@@ -261,7 +274,8 @@ __global__ void k_advance(unsigned long long *sweep_base, int n) {
 
 using PhaseFn = void (*)(const RepDev *, const int *, int, int, int, int,
                          const unsigned long long *, double *, double *, double *, const int *,
-                         int, const int *, double *, double *, double *, unsigned long long *);
+                         int, const int *, double *, double *, double *, unsigned int *,
+                         unsigned int *);
 
 PhaseFn phase_fn(int cap, int lpc) {
   if (cap == 8) return lpc == 64 ? k_phase<8, 64> : k_phase<8, 32>;
@@ -297,7 +311,7 @@ void enqueue_sweep(hsmc_gpu_ctx *ctx, int k, hipStream_t s, hipEvent_t *phase_ev
     hipLaunchKernelGGL(fn, dim3(blocks), dim3(lpc * cpb), lds, s, ctx->d_reps, ctx->d_aoff,
                        ctx->R, (int)ctx->Atot, p, k, ctx->d_sweep, ctx->d_x, ctx->d_y, ctx->d_z,
                        ctx->d_cnt, (int)ctx->Ctot, ctx->d_spid, ctx->d_sx, ctx->d_sy, ctx->d_sz,
-                       ctx->d_ctr);
+                       ctx->d_cacc, ctx->d_cleft);
     if (phase_events) (void)hipEventRecord(phase_events[2 * p + 1], s);
   }
 }
@@ -394,26 +408,27 @@ int hsmc_gpu_time_phase_kernel(hsmc_gpu_ctx *ctx, int n_sweeps, float *avg_ms, i
 
 int hsmc_gpu_counters(hsmc_gpu_ctx *ctx, int replica, uint64_t out[4]) {
   if (!ctx || replica < 0 || replica >= ctx->R || !out) return HSMC_ERR_BAD_ARG;
-  std::vector<unsigned long long> c(3 * kSpread);
+  int st = fold_counters(ctx);
+  if (st) return st;
   unsigned long long sw = 0;
-  HSMC_HIP(ctx, hipMemcpyAsync(c.data(), ctx->d_ctr + (size_t)replica * 3 * kSpread,
-                               sizeof(unsigned long long) * 3 * kSpread, hipMemcpyDeviceToHost,
-                               ctx->stream));
   HSMC_HIP(ctx, hipMemcpyAsync(&sw, ctx->d_sweep, sizeof(sw), hipMemcpyDeviceToHost, ctx->stream));
   HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  for (int kind = 0; kind < 3; ++kind) {
-    unsigned long long s = 0;
-    for (int i = 0; i < kSpread; ++i) s += c[kind * kSpread + i];
-    out[kind] = s;
-  }
+  // every occupant of every cell makes one attempt per sweep: N per sweep
+  out[0] = (uint64_t)ctx->reps[replica].p.n_part * (sw - ctx->sweep_at_reset);
+  out[1] = ctx->tot_acc[replica];
+  out[2] = ctx->tot_left[replica];
   out[3] = sw;
   return check_device_errors(ctx);
 }
 
 int hsmc_gpu_reset_counters(hsmc_gpu_ctx *ctx) {
   if (!ctx) return HSMC_ERR_BAD_ARG;
-  HSMC_HIP(ctx, hipMemsetAsync(ctx->d_ctr, 0, sizeof(unsigned long long) * ctx->R * 3 * kSpread,
-                               ctx->stream));
+  int st = fold_counters(ctx);
+  if (st) return st;
+  std::fill(ctx->tot_acc.begin(), ctx->tot_acc.end(), 0ull);
+  std::fill(ctx->tot_left.begin(), ctx->tot_left.end(), 0ull);
+  HSMC_HIP(ctx, hipMemcpyAsync(&ctx->sweep_at_reset, ctx->d_sweep, sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, ctx->stream));
   HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return HSMC_OK;
 }
@@ -438,13 +453,15 @@ int hsmc_gpu_autotune(hsmc_gpu_ctx *ctx, int sweeps_per_trial, hsmc_gpu_tuning *
   double *save = (double *)scratch(ctx, 23, sizeof(double) * 3 * P);
   if (!save) return HSMC_ERR_NO_MEMORY;
   unsigned long long sweep0 = 0;
-  std::vector<unsigned long long> ctr0((size_t)ctx->R * 3 * kSpread);
+  {
+    int st0 = fold_counters(ctx);
+    if (st0) return st0;
+  }
+  const std::vector<unsigned long long> acc0 = ctx->tot_acc, left0 = ctx->tot_left;
   HSMC_HIP(ctx, hipMemcpyAsync(save, ctx->d_x, sizeof(double) * P, hipMemcpyDeviceToDevice, ctx->stream));
   HSMC_HIP(ctx, hipMemcpyAsync(save + P, ctx->d_y, sizeof(double) * P, hipMemcpyDeviceToDevice, ctx->stream));
   HSMC_HIP(ctx, hipMemcpyAsync(save + 2 * P, ctx->d_z, sizeof(double) * P, hipMemcpyDeviceToDevice, ctx->stream));
   HSMC_HIP(ctx, hipMemcpyAsync(&sweep0, ctx->d_sweep, sizeof(sweep0), hipMemcpyDeviceToHost, ctx->stream));
-  HSMC_HIP(ctx, hipMemcpyAsync(ctr0.data(), ctx->d_ctr, sizeof(unsigned long long) * ctr0.size(),
-                               hipMemcpyDeviceToHost, ctx->stream));
   HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const hsmc_gpu_tuning start = ctx->tune;
   hsmc_gpu_tuning winner = start;
@@ -478,10 +495,12 @@ int hsmc_gpu_autotune(hsmc_gpu_ctx *ctx, int sweeps_per_trial, hsmc_gpu_tuning *
   HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_y, save + P, sizeof(double) * P, hipMemcpyDeviceToDevice, ctx->stream));
   HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_z, save + 2 * P, sizeof(double) * P, hipMemcpyDeviceToDevice, ctx->stream));
   HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_sweep, &sweep0, sizeof(sweep0), hipMemcpyHostToDevice, ctx->stream));
-  HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_ctr, ctr0.data(), sizeof(unsigned long long) * ctr0.size(),
-                               hipMemcpyHostToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cacc, 0, sizeof(unsigned int) * ctx->cell_alloc, ctx->stream));
+  HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cleft, 0, sizeof(unsigned int) * ctx->cell_alloc, ctx->stream));
   HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cnt, 0, sizeof(int) * 2 * ctx->cell_alloc, ctx->stream));
   HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->tot_acc = acc0;
+  ctx->tot_left = left0;
   if (best) *best = winner;
   return HSMC_OK;
 }
