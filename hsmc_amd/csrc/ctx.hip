@@ -194,6 +194,17 @@ __global__ void k_scale(double *__restrict__ x, double *__restrict__ y, double *
 
 bool valid_box(double L) { return L > 2.0 && std::isfinite(L); }
 
+__global__ void k_set_particles(double *__restrict__ x, double *__restrict__ y,
+                                double *__restrict__ z, const int *__restrict__ idx,
+                                const double *__restrict__ q, int m) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const int i = idx[k];
+  x[i] = q[3 * k];
+  y[i] = q[3 * k + 1];
+  z[i] = q[3 * k + 2];
+}
+
 }  // namespace
 
 extern "C" {
@@ -351,6 +362,32 @@ int hsmc_gpu_download(hsmc_gpu_ctx *ctx, int replica, double *conf4) {
   return check_device_errors(ctx);
 }
 
+int hsmc_gpu_set_particles(hsmc_gpu_ctx *ctx, int replica, const int32_t *idx,
+                           const double *xyz, int m) {
+  if (!ctx || replica < 0 || replica >= ctx->R || m < 0 || (m > 0 && (!idx || !xyz)))
+    return HSMC_ERR_BAD_ARG;
+  if (m == 0) return HSMC_OK;
+  const RepHost &h = ctx->reps[replica];
+  for (int k = 0; k < m; ++k) {
+    if (idx[k] < 0 || idx[k] >= h.p.n_part) return HSMC_ERR_BAD_ARG;
+    const double *q = xyz + 3 * (size_t)k;
+    if (!(q[0] >= 0.0 && q[0] <= h.p.lx && q[1] >= 0.0 && q[1] <= h.p.ly && q[2] >= 0.0 &&
+          q[2] <= h.p.lz))
+      return HSMC_ERR_BAD_ARG;
+  }
+  int *didx = (int *)scratch(ctx, 21, sizeof(int) * m);
+  double *dq = (double *)scratch(ctx, 22, sizeof(double) * 3 * m);
+  if (!didx || !dq) return HSMC_ERR_NO_MEMORY;
+  HSMC_HIP(ctx, hipMemcpyAsync(didx, idx, sizeof(int) * m, hipMemcpyHostToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(dq, xyz, sizeof(double) * 3 * m, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_set_particles, dim3((m + 255) / 256), dim3(256), 0, ctx->stream,
+                     ctx->d_x + h.poff, ctx->d_y + h.poff, ctx->d_z + h.poff, didx, dq, m);
+  HSMC_HIP(ctx, hipGetLastError());
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->version++;
+  return HSMC_OK;
+}
+
 int hsmc_gpu_set_box(hsmc_gpu_ctx *ctx, int replica, double lx, double ly, double lz) {
   if (!ctx || replica < 0 || replica >= ctx->R) return HSMC_ERR_BAD_ARG;
   if (!valid_box(lx) || !valid_box(ly) || !valid_box(lz)) return HSMC_ERR_BAD_ARG;
@@ -368,6 +405,12 @@ int hsmc_gpu_scale(hsmc_gpu_ctx *ctx, int replica, double sf, double lx, double 
                      ctx->d_x + h.poff, ctx->d_y + h.poff, ctx->d_z + h.poff, n, sf, lx, ly, lz);
   HSMC_HIP(ctx, hipGetLastError());
   h.p.lx = lx; h.p.ly = ly; h.p.lz = lz;
+  return layout(ctx);
+}
+
+int hsmc_gpu_set_dr_max(hsmc_gpu_ctx *ctx, int replica, double dr_max) {
+  if (!ctx || replica < 0 || replica >= ctx->R || !(dr_max >= 0.0)) return HSMC_ERR_BAD_ARG;
+  ctx->reps[replica].p.dr_max = dr_max;
   return layout(ctx);
 }
 

@@ -29,6 +29,7 @@ EXPORTS = [
     "hsmc_gpu_cell_list", "hsmc_gpu_overlap_batch", "hsmc_gpu_overlap_scaled",
     "hsmc_gpu_any_overlap_scaled", "hsmc_gpu_pair_dist", "hsmc_gpu_rdf_counts",
     "hsmc_gpu_contact_counts", "hsmc_gpu_widom_points", "hsmc_gpu_widom_uniform",
+    "hsmc_gpu_set_particles", "hsmc_gpu_set_dr_max",
 ]
 
 
@@ -76,7 +77,9 @@ def load() -> C.CDLL:
         "hsmc_gpu_upload": (i32, [vp, i32, P(f64)]),
         "hsmc_gpu_download": (i32, [vp, i32, P(f64)]),
         "hsmc_gpu_set_box": (i32, [vp, i32, f64, f64, f64]),
+        "hsmc_gpu_set_particles": (i32, [vp, i32, P(i32), P(f64), i32]),
         "hsmc_gpu_scale": (i32, [vp, i32, f64, f64, f64, f64]),
+        "hsmc_gpu_set_dr_max": (i32, [vp, i32, f64]),
         "hsmc_gpu_grid": (i32, [vp, i32, P(i32)]),
         "hsmc_gpu_sweeps": (i32, [vp, i32]),
         "hsmc_gpu_time_sweeps": (i32, [vp, i32, P(C.c_float)]),
@@ -159,6 +162,12 @@ class Context:
         out = np.zeros((self.n(r), 4))
         self._ck(self.L.hsmc_gpu_download(self.h, r, _p(out, C.c_double)), "download")
         return out
+
+    def set_particles(self, idx, xyz, r: int = 0) -> None:
+        i = np.ascontiguousarray(idx, dtype=np.int32)
+        q = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        self._ck(self.L.hsmc_gpu_set_particles(self.h, r, _p(i, C.c_int32), _p(q, C.c_double),
+                                               len(i)), "set_particles")
 
     def set_box(self, box, r: int = 0) -> None:
         self._ck(self.L.hsmc_gpu_set_box(self.h, r, *map(float, box)), "set_box")

@@ -84,6 +84,10 @@ int hsmc_gpu_sync(hsmc_gpu_ctx *ctx);
 int hsmc_gpu_upload(hsmc_gpu_ctx *ctx, int replica, const double *conf4);
 /* Copy the replica's configuration out as N x {id,x,y,z}. */
 int hsmc_gpu_download(hsmc_gpu_ctx *ctx, int replica, double *conf4);
+/* Overwrite the coordinates of m particles: idx[k] moves to
+ * xyz[3k..3k+2] (the part_conf update of an accepted move, moves.c:52-54). */
+int hsmc_gpu_set_particles(hsmc_gpu_ctx *ctx, int replica, const int32_t *idx,
+                           const double *xyz, int m);
 /* Replace the box edges without moving particles. */
 int hsmc_gpu_set_box(hsmc_gpu_ctx *ctx, int replica, double lx, double ly, double lz);
 /* Accepted volume move (moves.c:129-142): multiply every coordinate by sf,
@@ -91,6 +95,9 @@ int hsmc_gpu_set_box(hsmc_gpu_ctx *ctx, int replica, double lx, double ly, doubl
  * (moves.c:215-226).  The sweep grid is re-chosen for the new box. */
 int hsmc_gpu_scale(hsmc_gpu_ctx *ctx, int replica, double sf,
                    double lx, double ly, double lz);
+/* Set the displacement width of the replica (G_IN.dr_max, changed by the
+ * step-size optimizer, optimizer.c:37-53). */
+int hsmc_gpu_set_dr_max(hsmc_gpu_ctx *ctx, int replica, double dr_max);
 /* Sweep grid of the replica (cells per axis). */
 int hsmc_gpu_grid(const hsmc_gpu_ctx *ctx, int replica, int32_t out_g[3]);
 
