@@ -57,19 +57,46 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["n32000", "batched"], default="n32000",
+                    help="n32000: the headline single system; batched: 256 replicas "
+                         "N = 4 n^3, n in 3..15, rho in [0.30, 0.93]")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", type=str, default="",
+                    help="lanes_per_cell,cells_per_block,sweeps_per_graph (skips the autotuner)")
+    ap.add_argument("--no-phase-timing", action="store_true",
+                    help="skip the per-launch HIP-event timing pass (for profiler runs)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run one device-to-device copy of this many MiB (torch) and exit: "
+                         "the known-size reference for rocprofv3 FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.calibrate_copy_mib:
+        import torch
+        n = args.calibrate_copy_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"[bench] copied {args.calibrate_copy_mib} MiB", file=sys.stderr)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(independent replicas shard across GPUs, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batched":
+        return batched(args)
 
     from hsmc_amd import gpu
     conf, box = gpu.lattice(2, 20, 0.85)
     ctx = gpu.Context([dict(n=len(conf), box=box, dr_max=0.1, seed=1)])
     ctx.upload(conf)
     print(f"[bench] grid {ctx.grid()}  N={len(conf)}", file=sys.stderr, flush=True)
-    tuning = ctx.get_tuning() if args.no_autotune else ctx.autotune(64)
+    if args.tuning:
+        ctx.set_tuning(*(int(v) for v in args.tuning.split(",")))
+        tuning = ctx.get_tuning()
+    else:
+        tuning = ctx.get_tuning() if args.no_autotune else ctx.autotune(64)
     print(f"[bench] tuning {tuning}", file=sys.stderr, flush=True)
 
     ctx.sweeps(args.warmup)
@@ -87,7 +114,7 @@ def main() -> None:
           f"acceptance {c['accepted'] / c['attempts']:.4f}", file=sys.stderr, flush=True)
 
     # dominant kernel: the sublattice-phase kernel, timed per launch
-    avg_ms, launches = ctx.time_phase_kernel(200)
+    avg_ms, launches = (float("nan"), 0) if args.no_phase_timing else ctx.time_phase_kernel(200)
     per_launch_bytes = BYTES_PER_ATTEMPT * N_PART / 8.0
     achieved = per_launch_bytes / (avg_ms / 1e3) / 1e9
     if ctx.any_overlap_scaled([1.0])[0] != 1:
@@ -119,6 +146,48 @@ def main() -> None:
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line))
+
+
+def batched(args) -> None:
+    """256 independent replicas swept together (one graph launch per batch of
+    sweeps); prints its own JSON line."""
+    import numpy as np
+
+    from hsmc_amd import gpu
+    rng = np.random.default_rng(2026)
+    reps = []
+    for r in range(256):
+        n = int(rng.integers(3, 16))
+        rho = float(rng.uniform(0.30, 0.93))
+        conf, box = gpu.lattice(2, n, rho)
+        reps.append(dict(n=len(conf), box=box, dr_max=0.1, seed=1 + r, conf=conf))
+    total = sum(r["n"] for r in reps)
+    ctx = gpu.Context(reps)
+    for r, rep in enumerate(reps):
+        ctx.upload(rep["conf"], r)
+    tuning = ctx.autotune(32) if not args.tuning else None
+    if args.tuning:
+        ctx.set_tuning(*(int(v) for v in args.tuning.split(",")))
+        tuning = ctx.get_tuning()
+    print(f"[bench] batched: {len(reps)} replicas, {total} spheres, tuning {tuning}",
+          file=sys.stderr, flush=True)
+    ctx.sweeps(args.warmup)
+    ctx.sync()
+    steps = args.steps
+    ms = ctx.time_sweeps(steps)
+    value = total * steps / (ms / 1e3)
+    acc = [ctx.counters(r) for r in range(len(reps))]
+    print(json.dumps({
+        "metric": "MC move attempts/s (256 batched replicas, N=108..13500, rho=0.30..0.93)",
+        "value": value, "unit": "MC move attempts/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": ms / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (fcc lattice starts, Philox proposals)",
+        "config": {"workload": "256 NVT replicas, N = 4 n^3 with n uniform in 3..15, rho uniform "
+                               "in [0.30, 0.93], one Philox seed each", "total_spheres": total,
+                   "tuning": tuning,
+                   "acceptance": sum(a["accepted"] for a in acc) / max(1, sum(a["attempts"] for a in acc))},
+    }))
 
 
 if __name__ == "__main__":

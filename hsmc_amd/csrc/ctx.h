@@ -10,6 +10,7 @@
 //   sx, sy, sz       FP64[C*cap] position copy in each cell slot
 //   reps             RepDev[R]   box, grid and RNG key of each replica
 //   aoff             int32[R+1]  prefix of active cells per phase
+//   arep             int32[A]    replica owning each active cell
 //   sweep            uint64      sweeps done (read by every sweep kernel)
 //   cacc, cleft      uint32[C]   accepted moves / moves rejected for leaving
 //                                the cell, per sweep cell; each cell is
@@ -57,6 +58,9 @@ struct hsmc_gpu_ctx {
   size_t cell_alloc = 0;  // cells allocated (>= Ctot)
   hsmc::RepDev *d_reps = nullptr;
   int *d_aoff = nullptr;
+  int *d_arep = nullptr;  // replica of each active cell (Atot entries)
+  size_t arep_alloc = 0;
+  int lcap = 128;         // LDS candidate-list capacity of the phase kernel
   int *d_prep = nullptr;
   double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr;
   int *d_cnt = nullptr;

@@ -119,6 +119,36 @@ int layout(hsmc_gpu_ctx *ctx) {
   }
   ao[ctx->R] = (int)aoff;
   if (coff > (1L << 30)) return HSMC_ERR_NOT_SUPPORTED;
+  // Capacity of the phase kernel's LDS list of neighbour spheres: spheres of
+  // unit diameter whose centres lie in the 27-cell stencil fit in a box one
+  // diameter wider; their count is bounded by the close-packing fraction
+  // pi/(3 sqrt 2) = 0.7405 (taken as 0.78 for margin) times that volume over
+  // the sphere volume, and by N.
+  {
+    long tmax = 0;
+    for (int r = 0; r < ctx->R; ++r) {
+      const RepDev &d = rd[r];
+      const double span_x = std::min(3, d.gx) * d.sx + 1.0;
+      const double span_y = std::min(3, d.gy) * d.sy + 1.0;
+      const double span_z = std::min(3, d.gz) * d.sz + 1.0;
+      long b = (long)ceil(0.78 * span_x * span_y * span_z / (M_PI / 6.0));
+      if (b > d.N) b = d.N;
+      if (b > tmax) tmax = b;
+    }
+    ctx->lcap = tmax <= 128 ? 128 : (tmax <= 192 ? 192 : 256);
+  }
+  if ((size_t)aoff > ctx->arep_alloc) {
+    (void)hipFree(ctx->d_arep);
+    ctx->d_arep = nullptr;
+    HSMC_HIP(ctx, hipMalloc(&ctx->d_arep, sizeof(int) * (aoff + aoff / 8 + 64)));
+    ctx->arep_alloc = aoff + aoff / 8 + 64;
+  }
+  {
+    std::vector<int> arep(aoff);
+    for (int r = 0; r < ctx->R; ++r)
+      for (long a = ao[r]; a < ao[r + 1]; ++a) arep[a] = r;
+    HSMC_HIP(ctx, hipMemcpy(ctx->d_arep, arep.data(), sizeof(int) * aoff, hipMemcpyHostToDevice));
+  }
   ctx->Ctot = coff;
   ctx->Atot = aoff;
   const int cap = big ? 16 : 8;
@@ -142,11 +172,9 @@ int layout(hsmc_gpu_ctx *ctx) {
     ctx->cell_alloc = alloc;
     ctx->cap = cap;
   }
-  // keep the phase kernel's LDS staging (27*cap spheres per active cell)
-  // within 64 KiB per workgroup
+  // keep the phase kernel's LDS staging within 64 KiB per workgroup
   {
-    const size_t lmax = 27 * (size_t)ctx->cap;
-    const size_t per_cell = 3 * lmax * sizeof(double) + (2 * lmax + ctx->cap) * sizeof(int);
+    const size_t per_cell = 3 * (size_t)ctx->lcap * sizeof(double) + 3 * (size_t)ctx->cap * sizeof(int);
     while (ctx->tune.cells_per_block > 1 &&
            per_cell * (size_t)ctx->tune.cells_per_block > 64 * 1024)
       ctx->tune.cells_per_block >>= 1;
@@ -295,7 +323,7 @@ int hsmc_gpu_destroy(hsmc_gpu_ctx *ctx) {
   if (!ctx) return HSMC_OK;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   drop_graph(ctx);
-  void *bufs[] = {ctx->d_reps, ctx->d_aoff, ctx->d_prep, ctx->d_x, ctx->d_y, ctx->d_z,
+  void *bufs[] = {ctx->d_reps, ctx->d_aoff, ctx->d_arep, ctx->d_prep, ctx->d_x, ctx->d_y, ctx->d_z,
                   ctx->d_cnt, ctx->d_spid, ctx->d_sx, ctx->d_sy, ctx->d_sz, ctx->d_sweep,
                   ctx->d_cacc, ctx->d_cleft, ctx->d_err};
   for (void *b : bufs)

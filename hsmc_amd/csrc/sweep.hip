@@ -95,37 +95,31 @@ k_rebin(const RepDev *__restrict__ reps, const int *__restrict__ prep, int P, in
   }
 }
 
-template <int CAP, int LPC>
+template <int CAP, int LPC, int LCAP>
 __global__ void __launch_bounds__(256)
-k_phase(const RepDev *__restrict__ reps, const int *__restrict__ aoff, int R, int A, int phase,
+k_phase(const RepDev *__restrict__ reps, const int *__restrict__ arep, int R, int A, int phase,
         int k, const unsigned long long *__restrict__ sweep_base, double *__restrict__ x,
         double *__restrict__ y, double *__restrict__ z, const int *__restrict__ cnt, int C,
         const int *__restrict__ spid, double *__restrict__ sx, double *__restrict__ sy,
         double *__restrict__ sz, unsigned int *__restrict__ cacc,
-        unsigned int *__restrict__ cleft) {
-  constexpr int LMAX = 27 * CAP;
+        unsigned int *__restrict__ cleft, unsigned int *__restrict__ err) {
+  // LDS per active cell: positions of the spheres of its 27 neighbour cells
+  // (at most LCAP, a bound the host derives from the cell side), and slot,
+  // particle id and id order of its own occupants
   extern __shared__ double smem[];
   const int cpb = blockDim.x / LPC;
   const int lane = threadIdx.x & (LPC - 1);
   const int sub = threadIdx.x / LPC;
-  double *Lx = smem + (size_t)sub * (3 * LMAX);
-  double *Ly = Lx + LMAX;
-  double *Lz = Ly + LMAX;
-  int *Ls = reinterpret_cast<int *>(smem + (size_t)cpb * 3 * LMAX) + (size_t)sub * (2 * LMAX + CAP);
-  int *Lp = Ls + LMAX;
-  int *Lo = Lp + LMAX;
+  double *Lx = smem + (size_t)sub * (3 * LCAP);
+  double *Ly = Lx + LCAP;
+  double *Lz = Ly + LCAP;
+  int *Ss = reinterpret_cast<int *>(smem + (size_t)cpb * 3 * LCAP) + (size_t)sub * (3 * CAP);
+  int *Sp = Ss + CAP;
+  int *Lo = Sp + CAP;
 
   const int a = blockIdx.x * cpb + sub;
   if (a >= A) return;  // the whole lane group leaves together
-  int r = 0;
-  {
-    int lo = 0, hi = R;
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (aoff[mid] <= a) lo = mid; else hi = mid;
-    }
-    r = lo;
-  }
+  const int r = R == 1 ? 0 : arep[a];
   const RepDev rp = reps[r];
   const unsigned long long sweep = *sweep_base + (unsigned long long)k;
   const int *cntb = cnt + (size_t)(sweep & 1ull) * C;
@@ -166,6 +160,11 @@ k_phase(const RepDev *__restrict__ reps, const int *__restrict__ aoff, int R, in
   const int T = __shfl(incl, LPC - 1, LPC);
   const int nc = __shfl(nj, 13, LPC);
   const int oc = __shfl(off, 13, LPC);
+  if (nc == 0) return;  // empty active cell: nothing to move
+  if (T > LCAP) {        // cannot happen below the host's packing bound; reported
+    if (lane == 0) atomicOr(err, kErrListFull);
+    return;
+  }
 
   // lane t < nc draws attempt t: occupant rank (rng_get_int rule) and the
   // displacement (r - 0.5) * dr_max per axis, off the serial path below
@@ -196,16 +195,18 @@ k_phase(const RepDev *__restrict__ reps, const int *__restrict__ aoff, int R, in
     Lx[e] = sx[s];
     Ly[e] = sy[s];
     Lz[e] = sz[s];
-    Ls[e] = (int)s;
-    Lp[e] = spid[s];
+    if (lane == 13) {
+      Ss[q] = (int)s;
+      Sp[q] = spid[s];
+    }
   }
   wave_lds_sync();
   // occupants of the active cell in ascending particle id
   if (lane < nc) {
-    const int me = Lp[oc + lane];
+    const int me = Sp[lane];
     int rank = 0;
-    for (int m = 0; m < nc; ++m) rank += (Lp[oc + m] < me) ? 1 : 0;
-    Lo[rank] = oc + lane;
+    for (int m = 0; m < nc; ++m) rank += (Sp[m] < me) ? 1 : 0;
+    Lo[rank] = lane;
   }
   wave_lds_sync();
 
@@ -215,7 +216,7 @@ k_phase(const RepDev *__restrict__ reps, const int *__restrict__ aoff, int R, in
     const double ddx = __shfl(my_dx, t, LPC);
     const double ddy = __shfl(my_dy, t, LPC);
     const double ddz = __shfl(my_dz, t, LPC);
-    const int e = Lo[kk];
+    const int e = oc + Lo[kk];
     double nx = Lx[e] + ddx;
     double ny = Ly[e] + ddy;
     double nz = Lz[e] + ddz;
@@ -250,12 +251,12 @@ k_phase(const RepDev *__restrict__ reps, const int *__restrict__ aoff, int R, in
   // write the active cell's spheres back (slot copy and by-id arrays)
   if (lane < nc) {
     const int e = oc + lane;
-    const int s = Ls[e];
+    const int s = Ss[lane];
     const double px = Lx[e], py = Ly[e], pz = Lz[e];
     sx[s] = px;
     sy[s] = py;
     sz[s] = pz;
-    const int g = rp.poff + Lp[e];
+    const int g = rp.poff + Sp[lane];
     x[g] = px;
     y[g] = py;
     z[g] = pz;
@@ -275,22 +276,31 @@ __global__ void k_advance(unsigned long long *sweep_base, int n) {
 using PhaseFn = void (*)(const RepDev *, const int *, int, int, int, int,
                          const unsigned long long *, double *, double *, double *, const int *,
                          int, const int *, double *, double *, double *, unsigned int *,
-                         unsigned int *);
+                         unsigned int *, unsigned int *);
 
-PhaseFn phase_fn(int cap, int lpc) {
-  if (cap == 8) return lpc == 64 ? k_phase<8, 64> : k_phase<8, 32>;
-  return lpc == 64 ? k_phase<16, 64> : k_phase<16, 32>;
+template <int CAP, int LCAP>
+PhaseFn phase_fn_l(int lpc) {
+  return lpc == 64 ? k_phase<CAP, 64, LCAP> : k_phase<CAP, 32, LCAP>;
 }
 
-size_t phase_lds_bytes(int cap, int cpb) {
-  const size_t lmax = 27 * (size_t)cap;
-  return (size_t)cpb * (3 * lmax * sizeof(double) + (2 * lmax + cap) * sizeof(int));
+PhaseFn phase_fn(int cap, int lpc, int lcap) {
+  if (cap == 8) {
+    if (lcap <= 128) return phase_fn_l<8, 128>(lpc);
+    return lcap <= 192 ? phase_fn_l<8, 192>(lpc) : phase_fn_l<8, 256>(lpc);
+  }
+  if (lcap <= 128) return phase_fn_l<16, 128>(lpc);
+  if (lcap <= 192) return phase_fn_l<16, 192>(lpc);
+  return phase_fn_l<16, 256>(lpc);
+}
+
+size_t phase_lds_bytes(int cap, int lcap, int cpb) {
+  return (size_t)cpb * (3 * (size_t)lcap * sizeof(double) + 3 * (size_t)cap * sizeof(int));
 }
 
 bool tuning_ok(const hsmc_gpu_ctx *ctx, const hsmc_gpu_tuning &t) {
   if (t.lanes_per_cell != 32 && t.lanes_per_cell != 64) return false;
   if (t.cells_per_block < 1 || t.lanes_per_cell * t.cells_per_block > 256) return false;
-  if (phase_lds_bytes(ctx->cap, t.cells_per_block) > 64 * 1024) return false;
+  if (phase_lds_bytes(ctx->cap, ctx->lcap, t.cells_per_block) > 64 * 1024) return false;
   if (t.sweeps_per_graph < 0 || t.sweeps_per_graph > 512) return false;
   return true;
 }
@@ -304,15 +314,15 @@ void enqueue_sweep(hsmc_gpu_ctx *ctx, int k, hipStream_t s, hipEvent_t *phase_ev
                      ctx->d_sweep, k, ctx->d_err);
   const int lpc = ctx->tune.lanes_per_cell, cpb = ctx->tune.cells_per_block;
   const unsigned blocks = (unsigned)((ctx->Atot + cpb - 1) / cpb);
-  const size_t lds = phase_lds_bytes(ctx->cap, cpb);
-  PhaseFn fn = phase_fn(ctx->cap, lpc);
+  const size_t lds = phase_lds_bytes(ctx->cap, ctx->lcap, cpb);
+  PhaseFn fn = phase_fn(ctx->cap, lpc, ctx->lcap);
   for (int p = 0; p < 8; ++p) {
-    if (phase_events) (void)hipEventRecord(phase_events[2 * p], s);
-    hipLaunchKernelGGL(fn, dim3(blocks), dim3(lpc * cpb), lds, s, ctx->d_reps, ctx->d_aoff,
+    if (phase_events && p == 0) (void)hipEventRecord(phase_events[0], s);
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(lpc * cpb), lds, s, ctx->d_reps, ctx->d_arep,
                        ctx->R, (int)ctx->Atot, p, k, ctx->d_sweep, ctx->d_x, ctx->d_y, ctx->d_z,
                        ctx->d_cnt, (int)ctx->Ctot, ctx->d_spid, ctx->d_sx, ctx->d_sy, ctx->d_sz,
-                       ctx->d_cacc, ctx->d_cleft);
-    if (phase_events) (void)hipEventRecord(phase_events[2 * p + 1], s);
+                       ctx->d_cacc, ctx->d_cleft, ctx->d_err);
+    if (phase_events && p == 7) (void)hipEventRecord(phase_events[15], s);
   }
 }
 
@@ -385,6 +395,9 @@ int hsmc_gpu_time_sweeps(hsmc_gpu_ctx *ctx, int n_sweeps, float *ms) {
 
 int hsmc_gpu_time_phase_kernel(hsmc_gpu_ctx *ctx, int n_sweeps, float *avg_ms, int32_t *launches) {
   if (!ctx || n_sweeps <= 0 || !avg_ms) return HSMC_ERR_BAD_ARG;
+  // events bracket the 8 back-to-back phase launches of each sweep (the
+  // rebin kernel runs outside the bracket), so per-event overhead is paid
+  // once per 8 launches
   hipEvent_t ev[16];
   for (int i = 0; i < 16; ++i) HSMC_HIP(ctx, hipEventCreate(&ev[i]));
   double total = 0.0;
@@ -393,14 +406,13 @@ int hsmc_gpu_time_phase_kernel(hsmc_gpu_ctx *ctx, int n_sweeps, float *avg_ms, i
     enqueue_sweep(ctx, 0, ctx->stream, ev);
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, ctx->stream, ctx->d_sweep, 1);
     HSMC_HIP(ctx, hipEventSynchronize(ev[15]));
-    for (int p = 0; p < 8; ++p) {
-      float ms = 0.f;
-      HSMC_HIP(ctx, hipEventElapsedTime(&ms, ev[2 * p], ev[2 * p + 1]));
-      total += ms;
-      ++count;
-    }
+    float ms = 0.f;
+    HSMC_HIP(ctx, hipEventElapsedTime(&ms, ev[0], ev[15]));
+    total += ms;
+    count += 8;
   }
   for (int i = 0; i < 16; ++i) (void)hipEventDestroy(ev[i]);
+  ctx->version++;
   *avg_ms = (float)(total / count);
   if (launches) *launches = count;
   return check_device_errors(ctx);
