@@ -88,7 +88,7 @@ struct hsmc_gpu_ctx {
   unsigned long long aux_version = 0;
   int aux_n[3] = {0, 0, 0};
   // scratch buffers of the observables, grown on demand
-  static constexpr int kScratch = 24;
+  static constexpr int kScratch = 40;
   void *scratch[kScratch] = {};
   size_t scratch_bytes[kScratch] = {};
 };

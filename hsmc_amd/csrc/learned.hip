@@ -1,0 +1,971 @@
+// learned.hip -- learned Widom proposal (no counterpart in the reference,
+// whose insertions are uniform: compute_widom_chem_pot.c:44-80).
+//
+// The box is divided into M^3 voxels of side ~0.25 sigma.  Each voxel gets a
+// descriptor -- the sorted distances from its centre to its 12 nearest sphere
+// centres (FP32) -- and a score f_v from a 12->64->64->1 ReLU network whose
+// weights are BF16 and whose products run on the MFMA units with FP32
+// accumulation.  Insertions are drawn from q_v = mix * softmax(f)_v +
+// (1 - mix)/V, allocated to voxels by stratified rounding (voxel v receives
+// floor(n q_v) or one more, with mean exactly n q_v) and placed uniformly in
+// their voxel; the overlap test is the exact FP64 one.  The estimate
+// P = (1/(V n)) sum_hits 1/q_v of the insertion probability is unbiased for
+// any q with q_v > 0, so plain uniform insertion stays the reference.
+//
+// The network is trained on the GPU by the kernels below (forward, backward,
+// Adam) on uniform insertions: input the descriptor of the voxel holding the
+// point, target -beta * max(0, 1 - d_min) with d_min the point's distance to
+// the nearest centre.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "ctx.h"
+#include "device_common.h"
+#include "hsmc_gpu.h"
+
+using namespace hsmc;
+
+namespace {
+
+constexpr int NIN = HSMC_MLP_IN;       // 12 descriptor entries
+constexpr int KIN = 16;                // padded to the MFMA K
+constexpr int NH = HSMC_MLP_HIDDEN;    // 64
+constexpr int OW1 = 0, OB1 = OW1 + NIN * NH, OW2 = OB1 + NH, OB2 = OW2 + NH * NH,
+              OW3 = OB2 + NH, OB3 = OW3 + NH, NPARAM = OB3 + 1;
+static_assert(NPARAM == HSMC_MLP_NPARAM, "parameter layout");
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// round-to-nearest-even FP32 -> BF16 (finite inputs)
+__host__ __device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ __forceinline__ float bf2f(uint16_t h) {
+  return __builtin_bit_cast(float, (uint32_t)h << 16);
+}
+
+// Network as the kernels read it: BF16 weights for the MFMA layers.
+struct MlpDev {
+  uint16_t W1[KIN * NH];  // [k][n], rows NIN..KIN-1 zero
+  uint16_t W2[NH * NH];   // [in][out]
+  float b1[NH], b2[NH], w3[NH], b3, pad[3];
+};
+
+__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ bf16x4 ld4(const uint16_t *p) {  // 4 consecutive bf16
+  bf16x4 v;
+  v[0] = (short)p[0];
+  v[1] = (short)p[1];
+  v[2] = (short)p[2];
+  v[3] = (short)p[3];
+  return v;
+}
+__device__ __forceinline__ bf16x4 ld4s(const uint16_t *p, int stride) {  // strided
+  bf16x4 v;
+  v[0] = (short)p[0];
+  v[1] = (short)p[stride];
+  v[2] = (short)p[2 * stride];
+  v[3] = (short)p[3 * stride];
+  return v;
+}
+
+// Copy the network into LDS (whole block).
+__device__ void load_net(const MlpDev *__restrict__ net, MlpDev *s) {
+  const uint32_t *src = reinterpret_cast<const uint32_t *>(net);
+  uint32_t *dst = reinterpret_cast<uint32_t *>(s);
+  for (int i = threadIdx.x; i < (int)(sizeof(MlpDev) / 4); i += blockDim.x) dst[i] = src[i];
+  __syncthreads();
+}
+
+// Forward pass of one wave's 64 rows: X [64][KIN] bf16 in LDS -> H1 [64][NH]
+// bf16 in LDS -> layer 2 on MFMA -> y[64] (ys, LDS).  When H2 != nullptr the
+// post-ReLU layer-2 activations are kept (FP32) for the backward pass.
+// MFMA 16x16x16 operand layout: lane l feeds A row (l%16), k 4*(l/16)..+3;
+// B column (l%16), k 4*(l/16)..+3; receives D column (l%16), rows
+// 4*(l/16)..+3.
+__device__ void mlp_forward_wave(const MlpDev *s, const uint16_t *X, uint16_t *H1, float *H2,
+                                 float *ys) {
+  const int lane = threadIdx.x & 63, l16 = lane & 15, g4 = lane >> 4;
+  // layer 1: [64 x 16] x [16 x 64]
+  for (int R = 0; R < 4; ++R) {
+    const bf16x4 a = ld4(X + (R * 16 + l16) * KIN + 4 * g4);
+    for (int C = 0; C < 4; ++C) {
+      const bf16x4 b = ld4s(s->W1 + (4 * g4) * NH + C * 16 + l16, NH);
+      f32x4 d = {0.f, 0.f, 0.f, 0.f};
+      d = mfma16(a, b, d);
+      const float bias = s->b1[C * 16 + l16];
+      for (int i = 0; i < 4; ++i) {
+        const float h = fmaxf(d[i] + bias, 0.f);
+        H1[(R * 16 + 4 * g4 + i) * NH + C * 16 + l16] = f2bf(h);
+      }
+    }
+  }
+  wave_sync();
+  // layer 2 [64 x 64] x [64 x 64], then layer 3 (64 -> 1) on the VALU
+  for (int R = 0; R < 4; ++R) {
+    float part[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int C = 0; C < 4; ++C) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      for (int kt = 0; kt < 4; ++kt) {
+        const bf16x4 a = ld4(H1 + (R * 16 + l16) * NH + kt * 16 + 4 * g4);
+        const bf16x4 b = ld4s(s->W2 + (kt * 16 + 4 * g4) * NH + C * 16 + l16, NH);
+        acc = mfma16(a, b, acc);
+      }
+      const float bias = s->b2[C * 16 + l16], w = s->w3[C * 16 + l16];
+      for (int i = 0; i < 4; ++i) {
+        const float h = fmaxf(acc[i] + bias, 0.f);
+        if (H2) H2[(R * 16 + 4 * g4 + i) * NH + C * 16 + l16] = h;
+        part[i] += h * w;
+      }
+    }
+    for (int i = 0; i < 4; ++i) {
+      float v = part[i];
+      v += __shfl_xor(v, 1, 64);
+      v += __shfl_xor(v, 2, 64);
+      v += __shfl_xor(v, 4, 64);
+      v += __shfl_xor(v, 8, 64);
+      if (l16 == 0) ys[R * 16 + 4 * g4 + i] = v + s->b3;
+    }
+  }
+  wave_sync();
+}
+
+// ---- forward / backward over a batch of training rows ----------------------
+// One block = one wave = 64 rows.  Gradients are accumulated into grad
+// (FP32, packed like the parameters) with atomics; loss += sum (y - t)^2.
+__global__ void __launch_bounds__(64)
+k_mlp_train_step(const MlpDev *__restrict__ net, const float *__restrict__ Xg,
+                 const float *__restrict__ Tg, int n, float inv_batch, float *__restrict__ grad,
+                 float *__restrict__ loss, float *__restrict__ yout) {
+  __shared__ MlpDev s;
+  __shared__ uint16_t X[64 * KIN];
+  __shared__ uint16_t H1[64 * NH];
+  __shared__ float H2[64 * NH];  // reused for dH1 (bf16) after dH2 is formed
+  __shared__ uint16_t dH2[64 * NH];
+  __shared__ float ys[64], dys[64];
+  load_net(net, &s);
+  const int lane = threadIdx.x, l16 = lane & 15, g4 = lane >> 4;
+  const int row0 = blockIdx.x * 64;
+  const int row = row0 + lane;
+  const bool valid = row < n;
+  for (int k = 0; k < KIN; ++k)
+    X[lane * KIN + k] = (valid && k < NIN) ? f2bf(Xg[(size_t)row * NIN + k]) : (uint16_t)0;
+  wave_sync();
+  mlp_forward_wave(&s, X, H1, H2, ys);
+  // loss and dy = 2 (y - t) / B
+  {
+    const float y = ys[lane];
+    const float t = valid ? Tg[row] : y;
+    const float e = valid ? (y - t) : 0.f;
+    if (valid && yout) yout[row] = y;
+    dys[lane] = 2.f * e * inv_batch;
+    float sq = e * e;
+    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o, 64);
+    if (lane == 0) atomicAdd(loss, sq);
+  }
+  wave_sync();
+  // dW3[j] = sum_b H2[b][j] dy[b]; db3 = sum dy; dH2 = dy w3 (H2 > 0); db2
+  {
+    const int j = lane;
+    float g3 = 0.f, gb2 = 0.f;
+    const float w = s.w3[j];
+    for (int b = 0; b < 64; ++b) {
+      const float h = H2[b * NH + j];
+      const float dy = dys[b];
+      g3 += h * dy;
+      const float d = h > 0.f ? dy * w : 0.f;
+      gb2 += d;
+      dH2[b * NH + j] = f2bf(d);
+    }
+    atomicAdd(&grad[OW3 + j], g3);
+    atomicAdd(&grad[OB2 + j], gb2);
+    float sdy = dys[lane];
+    for (int o = 32; o > 0; o >>= 1) sdy += __shfl_xor(sdy, o, 64);
+    if (lane == 0) atomicAdd(&grad[OB3], sdy);
+  }
+  wave_sync();
+  // dW2 = H1^T dH2 (K = the 64 rows)
+  for (int Ri = 0; Ri < 4; ++Ri)
+    for (int Co = 0; Co < 4; ++Co) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      for (int kt = 0; kt < 4; ++kt) {
+        const bf16x4 a = ld4s(H1 + (kt * 16 + 4 * g4) * NH + Ri * 16 + l16, NH);
+        const bf16x4 b = ld4s(dH2 + (kt * 16 + 4 * g4) * NH + Co * 16 + l16, NH);
+        acc = mfma16(a, b, acc);
+      }
+      for (int i = 0; i < 4; ++i)
+        atomicAdd(&grad[OW2 + (Ri * 16 + 4 * g4 + i) * NH + Co * 16 + l16], acc[i]);
+    }
+  // dH1 = (dH2 W2^T) (H1 > 0), stored bf16 over the H2 buffer; db1
+  uint16_t *dH1 = reinterpret_cast<uint16_t *>(H2);
+  wave_sync();
+  for (int C = 0; C < 4; ++C) {
+    float colsum = 0.f;
+    for (int R = 0; R < 4; ++R) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      for (int kt = 0; kt < 4; ++kt) {
+        const bf16x4 a = ld4(dH2 + (R * 16 + l16) * NH + kt * 16 + 4 * g4);
+        const bf16x4 b = ld4(s.W2 + (C * 16 + l16) * NH + kt * 16 + 4 * g4);
+        acc = mfma16(a, b, acc);
+      }
+      for (int i = 0; i < 4; ++i) {
+        const int b = R * 16 + 4 * g4 + i, c = C * 16 + l16;
+        const float d = bf2f(H1[b * NH + c]) > 0.f ? acc[i] : 0.f;
+        colsum += d;
+        dH1[b * NH + c] = f2bf(d);
+      }
+    }
+    colsum += __shfl_xor(colsum, 16, 64);
+    colsum += __shfl_xor(colsum, 32, 64);
+    if (g4 == 0) atomicAdd(&grad[OB1 + C * 16 + l16], colsum);
+  }
+  wave_sync();
+  // dW1 = X^T dH1 (rows 0..11 of the 16 padded inputs)
+  for (int Co = 0; Co < 4; ++Co) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int kt = 0; kt < 4; ++kt) {
+      const bf16x4 a = ld4s(X + (kt * 16 + 4 * g4) * KIN + l16, KIN);
+      const bf16x4 b = ld4s(dH1 + (kt * 16 + 4 * g4) * NH + Co * 16 + l16, NH);
+      acc = mfma16(a, b, acc);
+    }
+    for (int i = 0; i < 4; ++i) {
+      const int f = 4 * g4 + i;
+      if (f < NIN) atomicAdd(&grad[OW1 + f * NH + Co * 16 + l16], acc[i]);
+    }
+  }
+}
+
+// Forward only over given rows (tests, scoring).
+__global__ void __launch_bounds__(64)
+k_mlp_forward(const MlpDev *__restrict__ net, const float *__restrict__ Xg, int n,
+              float *__restrict__ y) {
+  __shared__ MlpDev s;
+  __shared__ uint16_t X[64 * KIN];
+  __shared__ uint16_t H1[64 * NH];
+  __shared__ float ys[64];
+  load_net(net, &s);
+  const int lane = threadIdx.x;
+  const int row = blockIdx.x * 64 + lane;
+  for (int k = 0; k < KIN; ++k)
+    X[lane * KIN + k] = (row < n && k < NIN) ? f2bf(Xg[(size_t)row * NIN + k]) : (uint16_t)0;
+  wave_sync();
+  mlp_forward_wave(&s, X, H1, nullptr, ys);
+  if (row < n) y[row] = ys[lane];
+}
+
+// Adam on the FP32 master parameters, then the BF16 / FP32 kernel copies;
+// clears the gradient.
+__global__ void k_adam(float *__restrict__ P, float *__restrict__ G, float *__restrict__ m1,
+                       float *__restrict__ m2, float lr, float b1, float b2, float eps, float c1,
+                       float c2, MlpDev *__restrict__ net) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= NPARAM) return;
+  const float g = G[i];
+  G[i] = 0.f;
+  const float a = b1 * m1[i] + (1.f - b1) * g;
+  const float v = b2 * m2[i] + (1.f - b2) * g * g;
+  m1[i] = a;
+  m2[i] = v;
+  const float w = P[i] - lr * (a / c1) / (sqrtf(v / c2) + eps);
+  P[i] = w;
+  if (i < OB1) net->W1[i] = f2bf(w);
+  else if (i < OW2) net->b1[i - OB1] = w;
+  else if (i < OB2) net->W2[i - OW2] = f2bf(w);
+  else if (i < OW3) net->b2[i - OB2] = w;
+  else if (i < OB3) net->w3[i - OW3] = w;
+  else net->b3 = w;
+}
+
+__global__ void k_pack_net(const float *__restrict__ P, MlpDev *__restrict__ net) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < KIN * NH) net->W1[i] = (i < NIN * NH) ? f2bf(P[OW1 + i]) : (uint16_t)0;
+  if (i < NH * NH) net->W2[i] = f2bf(P[OW2 + i]);
+  if (i < NH) {
+    net->b1[i] = P[OB1 + i];
+    net->b2[i] = P[OB2 + i];
+    net->w3[i] = P[OW3 + i];
+  }
+  if (i == 0) net->b3 = P[OB3];
+}
+
+// ---- descriptors -------------------------------------------------------------
+struct NGrid {  // CSR observable grid (side >= 1.5) over one replica
+  int nx, ny, nz;
+  double sx, sy, sz, lx, ly, lz, hx, hy, hz;
+  const int *start;
+  const double *px, *py, *pz;
+};
+
+// 12 smallest squared distances (ascending) from q to sphere centres in the
+// 27 cells around q; entries not found stay at `far2`.
+__device__ __forceinline__ void nearest12(const NGrid &g, double qx, double qy, double qz,
+                                          float far2, float best[NIN]) {
+#pragma unroll
+  for (int k = 0; k < NIN; ++k) best[k] = far2;
+  const int cx = plain_cell(qx, g.sx, g.nx), cy = plain_cell(qy, g.sy, g.ny),
+            cz = plain_cell(qz, g.sz, g.nz);
+  for (int dx = -1; dx <= 1; ++dx) {
+    if (!stencil_axis_ok(dx, g.nx)) continue;
+    const int ax = wrap_index(cx + dx, g.nx);
+    for (int dy = -1; dy <= 1; ++dy) {
+      if (!stencil_axis_ok(dy, g.ny)) continue;
+      const int ay = wrap_index(cy + dy, g.ny);
+      for (int dz = -1; dz <= 1; ++dz) {
+        if (!stencil_axis_ok(dz, g.nz)) continue;
+        const int c = (ax * g.ny + ay) * g.nz + wrap_index(cz + dz, g.nz);
+        for (int p = g.start[c]; p < g.start[c + 1]; ++p) {
+          const float d2 = (float)dist2(qx, qy, qz, g.px[p], g.py[p], g.pz[p], g.lx, g.ly, g.lz,
+                                        g.hx, g.hy, g.hz);
+          if (d2 < best[NIN - 1]) {
+            best[NIN - 1] = d2;
+#pragma unroll
+            for (int k = NIN - 1; k > 0; --k) {
+              const float lo = fminf(best[k - 1], best[k]), hi = fmaxf(best[k - 1], best[k]);
+              best[k - 1] = lo;
+              best[k] = hi;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+struct Vox {
+  int mx, my, mz;
+  double vx, vy, vz;  // voxel edges
+  long V;
+};
+
+__device__ __forceinline__ void voxel_centre(const Vox &vx, long v, double &x, double &y,
+                                             double &z) {
+  const int iz = (int)(v % vx.mz);
+  const long t = v / vx.mz;
+  const int iy = (int)(t % vx.my), ix = (int)(t / vx.my);
+  x = (ix + 0.5) * vx.vx;
+  y = (iy + 0.5) * vx.vy;
+  z = (iz + 0.5) * vx.vz;
+}
+
+// descriptor entries: distances minus one diameter
+__device__ __forceinline__ void descriptor(const NGrid &g, double x, double y, double z,
+                                           float far2, float out[NIN]) {
+  float best[NIN];
+  nearest12(g, x, y, z, far2, best);
+#pragma unroll
+  for (int k = 0; k < NIN; ++k) out[k] = sqrtf(best[k]) - 1.0f;
+}
+
+__global__ void __launch_bounds__(256)
+k_point_descriptors(NGrid g, const double *__restrict__ q, int m, float far2,
+                    float *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  float d[NIN];
+  descriptor(g, q[3 * i], q[3 * i + 1], q[3 * i + 2], far2, d);
+  for (int k = 0; k < NIN; ++k) out[(size_t)i * NIN + k] = d[k];
+}
+
+// Training rows from uniform points of Philox stream (key, sample): input =
+// descriptor of the voxel holding the point, target = -beta * max(0, 1 - d).
+__global__ void __launch_bounds__(256)
+k_training_rows(NGrid g, Vox vx, uint32_t k0, uint32_t k1, unsigned long long sample, long long n,
+                float far2, float beta, float *__restrict__ X, float *__restrict__ T) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t u[4];
+  philox4x32_10((uint32_t)i, (uint32_t)((unsigned long long)i >> 32), (uint32_t)sample,
+                0x57490000u | (uint32_t)((sample >> 32) & 0xFFFFu), k0, k1, u);
+  const double qx = u32_unit_closed(u[0]) * g.lx, qy = u32_unit_closed(u[1]) * g.ly,
+               qz = u32_unit_closed(u[2]) * g.lz;
+  float best[NIN];
+  nearest12(g, qx, qy, qz, far2, best);
+  const float dmin = sqrtf(best[0]);
+  T[i] = -beta * fmaxf(0.f, 1.f - dmin);
+  int ix = (int)(qx / vx.vx), iy = (int)(qy / vx.vy), iz = (int)(qz / vx.vz);
+  ix = min(ix, vx.mx - 1);
+  iy = min(iy, vx.my - 1);
+  iz = min(iz, vx.mz - 1);
+  const long v = ((long)ix * vx.my + iy) * vx.mz + iz;
+  double cx, cy, cz;
+  voxel_centre(vx, v, cx, cy, cz);
+  float d[NIN];
+  descriptor(g, cx, cy, cz, far2, d);
+  for (int k = 0; k < NIN; ++k) X[(size_t)i * NIN + k] = d[k];
+}
+
+// Scores of all voxels: descriptor + network, fused (descriptors never leave
+// the chip).  Block = 4 waves = 256 voxels; per-block max and sum exp(f - max).
+__global__ void __launch_bounds__(256)
+k_voxel_scores(NGrid g, Vox vx, float far2, const MlpDev *__restrict__ net,
+               float *__restrict__ f, float *__restrict__ bmax, float *__restrict__ bsum) {
+  __shared__ MlpDev s;
+  __shared__ uint16_t X[4][64 * KIN];
+  __shared__ uint16_t H1[4][64 * NH];
+  __shared__ float ys[4][64];
+  __shared__ float red[8];
+  load_net(net, &s);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long v = (long)blockIdx.x * 256 + threadIdx.x;
+  float d[NIN];
+  if (v < vx.V) {
+    double cx, cy, cz;
+    voxel_centre(vx, v, cx, cy, cz);
+    descriptor(g, cx, cy, cz, far2, d);
+  } else {
+    for (int k = 0; k < NIN; ++k) d[k] = 0.f;
+  }
+  for (int k = 0; k < KIN; ++k) X[w][lane * KIN + k] = k < NIN ? f2bf(d[k]) : (uint16_t)0;
+  wave_sync();
+  mlp_forward_wave(&s, X[w], H1[w], nullptr, ys[w]);
+  const float val = ys[w][lane];
+  const bool ok = v < vx.V;
+  if (ok) f[v] = val;
+  float m = ok ? val : -INFINITY;
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if (lane == 0) red[w] = m;
+  __syncthreads();
+  const float bm = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  float e = ok ? expf(val - bm) : 0.f;
+  for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
+  __syncthreads();
+  if (lane == 0) red[4 + w] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    bmax[blockIdx.x] = bm;
+    bsum[blockIdx.x] = red[4] + red[5] + red[6] + red[7];
+  }
+}
+
+// Stratified allocation: voxel v gets floor(n q_v) insertions plus one with
+// probability frac(n q_v), so E[n_v] = n q_v exactly.
+__global__ void __launch_bounds__(256)
+k_allocate(const float *__restrict__ f, long V, double fmax, double sumexp, double mix, double n,
+           uint32_t k0, uint32_t k1, unsigned long long sample, int *__restrict__ cnt) {
+  const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  const double q = mix * exp((double)f[v] - fmax) / sumexp + (1.0 - mix) / (double)V;
+  const double nq = n * q;
+  const double fl = floor(nq);
+  uint32_t u[4];
+  philox4x32_10((uint32_t)v, (uint32_t)(v >> 32), (uint32_t)sample,
+                0x4C570000u | (uint32_t)((sample >> 32) & 0xFFFFu), k0, k1, u);
+  const double r = ((double)u[0] + 0.5) * 0x1p-32;
+  cnt[v] = (int)fl + (r < nq - fl ? 1 : 0);
+}
+
+__global__ void k_fill(const int *__restrict__ cnt, const int *__restrict__ off, long V,
+                       int *__restrict__ list) {
+  const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  const int c = cnt[v], o = off[v];
+  for (int j = 0; j < c; ++j) list[o + j] = (int)v;
+}
+
+__device__ __forceinline__ bool widom_free(const NGrid &g1, double qx, double qy, double qz) {
+  const int cx = plain_cell(qx, g1.sx, g1.nx), cy = plain_cell(qy, g1.sy, g1.ny),
+            cz = plain_cell(qz, g1.sz, g1.nz);
+  for (int dx = -1; dx <= 1; ++dx) {
+    if (!stencil_axis_ok(dx, g1.nx)) continue;
+    const int ax = wrap_index(cx + dx, g1.nx);
+    for (int dy = -1; dy <= 1; ++dy) {
+      if (!stencil_axis_ok(dy, g1.ny)) continue;
+      const int ay = wrap_index(cy + dy, g1.ny);
+      for (int dz = -1; dz <= 1; ++dz) {
+        if (!stencil_axis_ok(dz, g1.nz)) continue;
+        const int c = (ax * g1.ny + ay) * g1.nz + wrap_index(cz + dz, g1.nz);
+        for (int p = g1.start[c]; p < g1.start[c + 1]; ++p)
+          if (dist2(qx, qy, qz, g1.px[p], g1.py[p], g1.pz[p], g1.lx, g1.ly, g1.lz, g1.hx, g1.hy,
+                    g1.hz) < 1.0)
+            return false;
+      }
+    }
+  }
+  return true;
+}
+
+// Insertion i goes to voxel list[i], uniformly inside it; a hit adds its
+// importance weight 1/(V n q_v).
+__global__ void __launch_bounds__(256)
+k_insert(NGrid g1, Vox vx, const int *__restrict__ list, long total, const float *__restrict__ f,
+         double fmax, double sumexp, double mix, double n, uint32_t k0, uint32_t k1,
+         unsigned long long sample, double *__restrict__ acc, unsigned long long *__restrict__ hits) {
+  __shared__ double rs[4];
+  __shared__ unsigned long long rh[4];
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double w = 0.0;
+  unsigned long long h = 0;
+  if (i < total) {
+    const long v = list[i];
+    const int iz = (int)(v % vx.mz);
+    const long t = v / vx.mz;
+    const int iy = (int)(t % vx.my), ix = (int)(t / vx.my);
+    uint32_t u[4];
+    philox4x32_10((uint32_t)i, (uint32_t)(i >> 32), (uint32_t)sample,
+                  0x4C490000u | (uint32_t)((sample >> 32) & 0xFFFFu), k0, k1, u);
+    const double qx = (ix + ((double)u[0] + 0.5) * 0x1p-32) * vx.vx;
+    const double qy = (iy + ((double)u[1] + 0.5) * 0x1p-32) * vx.vy;
+    const double qz = (iz + ((double)u[2] + 0.5) * 0x1p-32) * vx.vz;
+    if (widom_free(g1, qx, qy, qz)) {
+      const double q = mix * exp((double)f[v] - fmax) / sumexp + (1.0 - mix) / (double)vx.V;
+      w = 1.0 / ((double)vx.V * n * q);
+      h = 1;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    w += __shfl_xor(w, o, 64);
+    h += __shfl_xor(h, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    rs[threadIdx.x >> 6] = w;
+    rh[threadIdx.x >> 6] = h;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double sw = rs[0] + rs[1] + rs[2] + rs[3];
+    const unsigned long long sh = rh[0] + rh[1] + rh[2] + rh[3];
+    if (sh) {
+      atomicAdd(acc, sw);
+      atomicAdd(hits, sh);
+    }
+  }
+}
+
+// exclusive prefix sum of cnt[0..n) (int) into off[0..n], multi-block
+__global__ void __launch_bounds__(1024) k_scan_local(const int *__restrict__ in, long n,
+                                                     int *__restrict__ out, int *__restrict__ bsum) {
+  __shared__ int s[1024];
+  const long base = (long)blockIdx.x * 4096;
+  int v[4], tot = 0;
+  for (int j = 0; j < 4; ++j) {
+    const long i = base + threadIdx.x * 4 + j;
+    v[j] = i < n ? in[i] : 0;
+    tot += v[j];
+  }
+  s[threadIdx.x] = tot;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const int t = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int run = s[threadIdx.x] - tot;
+  for (int j = 0; j < 4; ++j) {
+    const long i = base + threadIdx.x * 4 + j;
+    if (i < n) out[i] = run;
+    run += v[j];
+  }
+  if (threadIdx.x == 1023) bsum[blockIdx.x] = s[1023];
+}
+
+__global__ void k_scan_add(int *__restrict__ out, long n, const int *__restrict__ boff) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += boff[i / 4096];
+}
+
+// single-block exclusive scan of up to a few thousand entries (+ total)
+__global__ void __launch_bounds__(1024) k_scan_small(const int *__restrict__ in, int n,
+                                                     int *__restrict__ out) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x, per = (n + 1023) / 1024, b = t * per, e = min(n, b + per);
+  int s = 0;
+  for (int i = b; i < e; ++i) s += in[i];
+  part[t] = s;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const int v = t >= d ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+  for (int i = b; i < e; ++i) {
+    out[i] = run;
+    run += in[i];
+  }
+  if (t == 1023) out[n] = part[1023];
+}
+
+// Build a CSR grid of one replica (side >= cut) into the given scratch slots.
+int build_ngrid(hsmc_gpu_ctx *ctx, int replica, double cut, int slot0, NGrid &g);
+
+}  // namespace
+
+// ---- network object ----------------------------------------------------------
+struct hsmc_gpu_mlp {
+  hsmc_gpu_ctx *ctx = nullptr;
+  float *P = nullptr, *G = nullptr, *m1 = nullptr, *m2 = nullptr;  // FP32 master, grad, Adam
+  MlpDev *net = nullptr;
+  float *loss = nullptr;
+  long step = 0;
+};
+
+namespace {
+
+__global__ void k_grid_count2(const double *__restrict__ x, const double *__restrict__ y,
+                              const double *__restrict__ z, int n, int nx, int ny, int nz,
+                              double sx, double sy, double sz, int *__restrict__ cnt,
+                              int *__restrict__ cellof) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = (plain_cell(x[i], sx, nx) * ny + plain_cell(y[i], sy, ny)) * nz + plain_cell(z[i], sz, nz);
+  cellof[i] = c;
+  atomicAdd(&cnt[c], 1);
+}
+
+__global__ void k_grid_scatter2(const double *__restrict__ x, const double *__restrict__ y,
+                                const double *__restrict__ z, int n, const int *__restrict__ cellof,
+                                const int *__restrict__ start, int *__restrict__ fill,
+                                double *__restrict__ px, double *__restrict__ py,
+                                double *__restrict__ pz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cellof[i];
+  const int pos = start[c] + atomicAdd(&fill[c], 1);
+  px[pos] = x[i];
+  py[pos] = y[i];
+  pz[pos] = z[i];
+}
+
+int grid_cells_for2(double L, double cut) {
+  int n = (int)floor(L / cut);
+  if (n < 1) n = 1;
+  while (n > 1 && L / n < cut) --n;
+  return n;
+}
+
+// scratch slots 24..39 belong to this file (observables.hip uses 0..12,
+// ctx.hip 21..22, sweep.hip 23): descriptor grid 24..26, overlap grid 27..29,
+// rows / scores 30, targets / block partials 31, counts 32, offsets 33,
+// sums 34, insertion list 35, scan partials 36..37.
+enum { L_GRID_A = 24, L_GRID_B = 27, L_X = 30, L_T = 31, L_CNT = 32, L_OFF = 33, L_ACC = 34,
+       L_LIST = 35, L_BSUM = 36, L_BOFF = 37 };
+
+int build_ngrid(hsmc_gpu_ctx *ctx, int replica, double cut, int slot0, NGrid &g) {
+  const RepHost &h = ctx->reps[replica];
+  const int N = h.p.n_part;
+  const int nx = grid_cells_for2(h.p.lx, cut), ny = grid_cells_for2(h.p.ly, cut),
+            nz = grid_cells_for2(h.p.lz, cut);
+  const long C = (long)nx * ny * nz;
+  int *cnt = (int *)scratch(ctx, slot0 + 0, sizeof(int) * 2 * (C + 1) + sizeof(int) * N);
+  double *pos = (double *)scratch(ctx, slot0 + 1, sizeof(double) * 3 * N);
+  int *start = (int *)scratch(ctx, slot0 + 2, sizeof(int) * (C + 1));
+  if (!cnt || !pos || !start) return HSMC_ERR_NO_MEMORY;
+  int *fill = cnt + (C + 1);
+  int *cellof = fill + (C + 1);
+  const double *x = ctx->d_x + h.poff, *y = ctx->d_y + h.poff, *z = ctx->d_z + h.poff;
+  HSMC_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(int) * 2 * (C + 1), ctx->stream));
+  const double sx = h.p.lx / nx, sy = h.p.ly / ny, sz = h.p.lz / nz;
+  hipLaunchKernelGGL(k_grid_count2, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, x, y, z, N,
+                     nx, ny, nz, sx, sy, sz, cnt, cellof);
+  hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, ctx->stream, cnt, (int)C, start);
+  hipLaunchKernelGGL(k_grid_scatter2, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, x, y, z,
+                     N, cellof, start, fill, pos, pos + N, pos + 2 * (size_t)N);
+  HSMC_HIP(ctx, hipGetLastError());
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  g.sx = sx; g.sy = sy; g.sz = sz;
+  g.lx = h.p.lx; g.ly = h.p.ly; g.lz = h.p.lz;
+  g.hx = h.p.lx / 2.0; g.hy = h.p.ly / 2.0; g.hz = h.p.lz / 2.0;
+  g.start = start;
+  g.px = pos; g.py = pos + N; g.pz = pos + 2 * (size_t)N;
+  return HSMC_OK;
+}
+
+constexpr double kDescCut = 1.5;  // descriptor grid side (sigma)
+
+Vox make_vox(const RepHost &h, double voxel) {
+  Vox v;
+  v.mx = (int)ceil(h.p.lx / voxel);
+  v.my = (int)ceil(h.p.ly / voxel);
+  v.mz = (int)ceil(h.p.lz / voxel);
+  v.vx = h.p.lx / v.mx;
+  v.vy = h.p.ly / v.my;
+  v.vz = h.p.lz / v.mz;
+  v.V = (long)v.mx * v.my * v.mz;
+  return v;
+}
+
+float far2_of(const NGrid &g) {
+  const double s = std::min(g.sx, std::min(g.sy, g.sz));
+  return (float)(s * s);
+}
+
+int exclusive_scan(hsmc_gpu_ctx *ctx, const int *cnt, long V, int *off, int *total_out) {
+  const long nb = (V + 4095) / 4096;
+  int *bsum = (int *)scratch(ctx, L_BSUM, sizeof(int) * (nb + 1));
+  int *boff = (int *)scratch(ctx, L_BOFF, sizeof(int) * (nb + 1));
+  if (!bsum || !boff) return HSMC_ERR_NO_MEMORY;
+  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(1024), 0, ctx->stream, cnt, V, off, bsum);
+  hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, ctx->stream, bsum, (int)nb, boff);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, ctx->stream, off, V, boff);
+  HSMC_HIP(ctx, hipGetLastError());
+  HSMC_HIP(ctx, hipMemcpyAsync(total_out, boff + nb, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HSMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hsmc_gpu_mlp_create(hsmc_gpu_ctx *ctx, hsmc_gpu_mlp **out, uint64_t seed) {
+  if (!ctx || !out) return HSMC_ERR_BAD_ARG;
+  hsmc_gpu_mlp *m = new hsmc_gpu_mlp();
+  m->ctx = ctx;
+  if (hipMalloc(&m->P, sizeof(float) * NPARAM) != hipSuccess ||
+      hipMalloc(&m->G, sizeof(float) * NPARAM) != hipSuccess ||
+      hipMalloc(&m->m1, sizeof(float) * NPARAM) != hipSuccess ||
+      hipMalloc(&m->m2, sizeof(float) * NPARAM) != hipSuccess ||
+      hipMalloc(&m->net, sizeof(MlpDev)) != hipSuccess ||
+      hipMalloc(&m->loss, sizeof(float)) != hipSuccess) {
+    hsmc_gpu_mlp_destroy(m);
+    return HSMC_ERR_NO_MEMORY;
+  }
+  // He-uniform initial weights from Philox(seed), zero biases
+  std::vector<float> p(NPARAM, 0.f);
+  auto uni = [&](uint32_t i) {
+    uint32_t u[4];
+    philox4x32_10(i, 0x4D4C5000u, 0u, 0u, (uint32_t)seed, (uint32_t)(seed >> 32), u);
+    return (float)(((double)u[0] + 0.5) * 0x1p-32 * 2.0 - 1.0);
+  };
+  const float a1 = sqrtf(6.f / NIN), a2 = sqrtf(6.f / NH);
+  for (int i = 0; i < NIN * NH; ++i) p[OW1 + i] = a1 * uni(i);
+  for (int i = 0; i < NH * NH; ++i) p[OW2 + i] = a2 * uni(10000 + i);
+  for (int i = 0; i < NH; ++i) p[OW3 + i] = a2 * uni(20000 + i);
+  int st = hsmc_gpu_mlp_set(m, p.data());
+  if (st) {
+    hsmc_gpu_mlp_destroy(m);
+    return st;
+  }
+  *out = m;
+  return HSMC_OK;
+}
+
+int hsmc_gpu_mlp_destroy(hsmc_gpu_mlp *m) {
+  if (!m) return HSMC_OK;
+  if (m->ctx) (void)hipStreamSynchronize(m->ctx->stream);
+  void *b[] = {m->P, m->G, m->m1, m->m2, m->net, m->loss};
+  for (void *x : b)
+    if (x) (void)hipFree(x);
+  delete m;
+  return HSMC_OK;
+}
+
+int hsmc_gpu_mlp_set(hsmc_gpu_mlp *m, const float *params) {
+  if (!m || !params) return HSMC_ERR_BAD_ARG;
+  hsmc_gpu_ctx *ctx = m->ctx;
+  HSMC_HIP(ctx, hipMemcpyAsync(m->P, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemsetAsync(m->G, 0, sizeof(float) * NPARAM, ctx->stream));
+  HSMC_HIP(ctx, hipMemsetAsync(m->m1, 0, sizeof(float) * NPARAM, ctx->stream));
+  HSMC_HIP(ctx, hipMemsetAsync(m->m2, 0, sizeof(float) * NPARAM, ctx->stream));
+  HSMC_HIP(ctx, hipMemsetAsync(m->net, 0, sizeof(MlpDev), ctx->stream));
+  hipLaunchKernelGGL(k_pack_net, dim3((NH * NH + 255) / 256), dim3(256), 0, ctx->stream, m->P, m->net);
+  HSMC_HIP(ctx, hipGetLastError());
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  m->step = 0;
+  return HSMC_OK;
+}
+
+int hsmc_gpu_mlp_get(hsmc_gpu_mlp *m, float *params) {
+  if (!m || !params) return HSMC_ERR_BAD_ARG;
+  hsmc_gpu_ctx *ctx = m->ctx;
+  HSMC_HIP(ctx, hipMemcpyAsync(params, m->P, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HSMC_OK;
+}
+
+int hsmc_gpu_mlp_forward(hsmc_gpu_mlp *m, const float *x, int n, float *y) {
+  if (!m || n < 0 || (n > 0 && (!x || !y))) return HSMC_ERR_BAD_ARG;
+  if (n == 0) return HSMC_OK;
+  hsmc_gpu_ctx *ctx = m->ctx;
+  float *dx = (float *)scratch(ctx, L_X, sizeof(float) * NIN * (size_t)n);
+  float *dy = (float *)scratch(ctx, L_T, sizeof(float) * (size_t)n);
+  if (!dx || !dy) return HSMC_ERR_NO_MEMORY;
+  HSMC_HIP(ctx, hipMemcpyAsync(dx, x, sizeof(float) * NIN * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_mlp_forward, dim3((n + 63) / 64), dim3(64), 0, ctx->stream, m->net, dx, n, dy);
+  HSMC_HIP(ctx, hipGetLastError());
+  HSMC_HIP(ctx, hipMemcpyAsync(y, dy, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HSMC_OK;
+}
+
+int hsmc_gpu_mlp_gradient(hsmc_gpu_mlp *m, const float *x, const float *t, int n, float *loss,
+                          float *grad) {
+  if (!m || n <= 0 || !x || !t) return HSMC_ERR_BAD_ARG;
+  hsmc_gpu_ctx *ctx = m->ctx;
+  float *dx = (float *)scratch(ctx, L_X, sizeof(float) * NIN * (size_t)n);
+  float *dt = (float *)scratch(ctx, L_T, sizeof(float) * (size_t)n);
+  if (!dx || !dt) return HSMC_ERR_NO_MEMORY;
+  HSMC_HIP(ctx, hipMemcpyAsync(dx, x, sizeof(float) * NIN * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(dt, t, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemsetAsync(m->G, 0, sizeof(float) * NPARAM, ctx->stream));
+  HSMC_HIP(ctx, hipMemsetAsync(m->loss, 0, sizeof(float), ctx->stream));
+  hipLaunchKernelGGL(k_mlp_train_step, dim3((n + 63) / 64), dim3(64), 0, ctx->stream, m->net, dx,
+                     dt, n, 1.0f / n, m->G, m->loss, nullptr);
+  HSMC_HIP(ctx, hipGetLastError());
+  float l = 0.f;
+  HSMC_HIP(ctx, hipMemcpyAsync(&l, m->loss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  if (grad)
+    HSMC_HIP(ctx, hipMemcpyAsync(grad, m->G, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipMemsetAsync(m->G, 0, sizeof(float) * NPARAM, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (loss) *loss = l / n;
+  return HSMC_OK;
+}
+
+int hsmc_gpu_point_descriptors(hsmc_gpu_ctx *ctx, int replica, const double *xyz, int m,
+                               float *desc) {
+  if (!ctx || replica < 0 || replica >= ctx->R || m < 0 || (m > 0 && (!xyz || !desc)))
+    return HSMC_ERR_BAD_ARG;
+  if (m == 0) return HSMC_OK;
+  NGrid g;
+  int st = build_ngrid(ctx, replica, kDescCut, L_GRID_A, g);
+  if (st) return st;
+  double *dq = (double *)scratch(ctx, L_X, sizeof(double) * 3 * (size_t)m);
+  float *dd = (float *)scratch(ctx, L_T, sizeof(float) * NIN * (size_t)m);
+  if (!dq || !dd) return HSMC_ERR_NO_MEMORY;
+  HSMC_HIP(ctx, hipMemcpyAsync(dq, xyz, sizeof(double) * 3 * m, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_point_descriptors, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, g, dq,
+                     m, far2_of(g), dd);
+  HSMC_HIP(ctx, hipGetLastError());
+  HSMC_HIP(ctx, hipMemcpyAsync(desc, dd, sizeof(float) * NIN * m, hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HSMC_OK;
+}
+
+int hsmc_gpu_mlp_train(hsmc_gpu_mlp *m, int replica, double voxel, uint64_t seed, uint64_t sample,
+                       int64_t n_points, int epochs, int batch, float lr, float beta,
+                       float *final_loss) {
+  if (!m || n_points <= 0 || epochs <= 0 || batch < 64 || !(voxel > 0.0)) return HSMC_ERR_BAD_ARG;
+  hsmc_gpu_ctx *ctx = m->ctx;
+  if (replica < 0 || replica >= ctx->R) return HSMC_ERR_BAD_ARG;
+  NGrid g;
+  int st = build_ngrid(ctx, replica, kDescCut, L_GRID_A, g);
+  if (st) return st;
+  const Vox vx = make_vox(ctx->reps[replica], voxel);
+  float *X = (float *)scratch(ctx, L_X, sizeof(float) * NIN * (size_t)n_points);
+  float *T = (float *)scratch(ctx, L_T, sizeof(float) * (size_t)n_points);
+  if (!X || !T) return HSMC_ERR_NO_MEMORY;
+  hipLaunchKernelGGL(k_training_rows, dim3((unsigned)((n_points + 255) / 256)), dim3(256), 0,
+                     ctx->stream, g, vx, (uint32_t)seed, (uint32_t)(seed >> 32),
+                     (unsigned long long)sample, (long long)n_points, far2_of(g), beta, X, T);
+  HSMC_HIP(ctx, hipGetLastError());
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  float last = 0.f;
+  for (int e = 0; e < epochs; ++e) {
+    HSMC_HIP(ctx, hipMemsetAsync(m->loss, 0, sizeof(float), ctx->stream));
+    for (int64_t b0 = 0; b0 < n_points; b0 += batch) {
+      const int nb = (int)std::min<int64_t>(batch, n_points - b0);
+      hipLaunchKernelGGL(k_mlp_train_step, dim3((nb + 63) / 64), dim3(64), 0, ctx->stream, m->net,
+                         X + (size_t)b0 * NIN, T + b0, nb, 1.0f / nb, m->G, m->loss, nullptr);
+      ++m->step;
+      const float c1 = 1.f - powf(b1, (float)m->step), c2 = 1.f - powf(b2, (float)m->step);
+      hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, ctx->stream, m->P, m->G,
+                         m->m1, m->m2, lr, b1, b2, eps, c1, c2, m->net);
+    }
+    HSMC_HIP(ctx, hipGetLastError());
+    HSMC_HIP(ctx, hipMemcpyAsync(&last, m->loss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  if (final_loss) *final_loss = last / (float)n_points;
+  return HSMC_OK;
+}
+
+int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mix, uint64_t seed,
+                           uint64_t sample, int64_t n, double *estimate, int64_t *n_done,
+                           int64_t *hits) {
+  if (!m || !estimate || n <= 0 || n > (1LL << 30) || !(mix >= 0.0 && mix < 1.0) || !(voxel > 0.0))
+    return HSMC_ERR_BAD_ARG;
+  hsmc_gpu_ctx *ctx = m->ctx;
+  if (replica < 0 || replica >= ctx->R) return HSMC_ERR_BAD_ARG;
+  NGrid g, g1;
+  int st = build_ngrid(ctx, replica, kDescCut, L_GRID_A, g);
+  if (st) return st;
+  st = build_ngrid(ctx, replica, 1.0 + 1e-9, L_GRID_B, g1);
+  if (st) return st;
+  const Vox vx = make_vox(ctx->reps[replica], voxel);
+  if (vx.V > (1L << 31) - 1) return HSMC_ERR_NOT_SUPPORTED;
+  const long nblk = (vx.V + 255) / 256;
+  float *f = (float *)scratch(ctx, L_X, sizeof(float) * (size_t)vx.V);
+  float *bred = (float *)scratch(ctx, L_T, sizeof(float) * 2 * (size_t)nblk);
+  int *cnt = (int *)scratch(ctx, L_CNT, sizeof(int) * (size_t)vx.V);
+  int *off = (int *)scratch(ctx, L_OFF, sizeof(int) * (size_t)(vx.V + 1));
+  double *acc = (double *)scratch(ctx, L_ACC, sizeof(double) * 2);
+  if (!f || !bred || !cnt || !off || !acc) return HSMC_ERR_NO_MEMORY;
+  hipLaunchKernelGGL(k_voxel_scores, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, g, vx,
+                     far2_of(g), m->net, f, bred, bred + nblk);
+  HSMC_HIP(ctx, hipGetLastError());
+  std::vector<float> hb(2 * nblk);
+  HSMC_HIP(ctx, hipMemcpyAsync(hb.data(), bred, sizeof(float) * 2 * nblk, hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  double fmax = -INFINITY;
+  for (long b = 0; b < nblk; ++b) fmax = std::max(fmax, (double)hb[b]);
+  double sumexp = 0.0;
+  for (long b = 0; b < nblk; ++b) sumexp += (double)hb[nblk + b] * exp((double)hb[b] - fmax);
+  hipLaunchKernelGGL(k_allocate, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, f, vx.V, fmax,
+                     sumexp, mix, (double)n, (uint32_t)seed, (uint32_t)(seed >> 32),
+                     (unsigned long long)sample, cnt);
+  HSMC_HIP(ctx, hipGetLastError());
+  int total = 0;
+  st = exclusive_scan(ctx, cnt, vx.V, off, &total);
+  if (st) return st;
+  int *list = (int *)scratch(ctx, L_LIST, sizeof(int) * (size_t)(total + 1));
+  if (!list) return HSMC_ERR_NO_MEMORY;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, cnt, off, vx.V, list);
+  HSMC_HIP(ctx, hipMemsetAsync(acc, 0, sizeof(double) * 2, ctx->stream));
+  if (total > 0)
+    hipLaunchKernelGGL(k_insert, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream,
+                       g1, vx, list, (long)total, f, fmax, sumexp, mix, (double)n, (uint32_t)seed,
+                       (uint32_t)(seed >> 32), (unsigned long long)sample, acc,
+                       reinterpret_cast<unsigned long long *>(acc + 1));
+  HSMC_HIP(ctx, hipGetLastError());
+  double ha[2] = {0, 0};
+  HSMC_HIP(ctx, hipMemcpyAsync(ha, acc, sizeof(double) * 2, hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *estimate = ha[0];
+  if (n_done) *n_done = total;
+  if (hits) *hits = (int64_t)__builtin_bit_cast(unsigned long long, ha[1]);
+  return HSMC_OK;
+}
+
+int hsmc_gpu_voxel_scores(hsmc_gpu_mlp *m, int replica, double voxel, float *scores, int64_t cap,
+                          int64_t *n_voxels) {
+  if (!m || !(voxel > 0.0) || !n_voxels) return HSMC_ERR_BAD_ARG;
+  hsmc_gpu_ctx *ctx = m->ctx;
+  if (replica < 0 || replica >= ctx->R) return HSMC_ERR_BAD_ARG;
+  const Vox vx = make_vox(ctx->reps[replica], voxel);
+  *n_voxels = vx.V;
+  if (!scores) return HSMC_OK;
+  if (cap < vx.V) return HSMC_ERR_BAD_ARG;
+  NGrid g;
+  int st = build_ngrid(ctx, replica, kDescCut, L_GRID_A, g);
+  if (st) return st;
+  const long nblk = (vx.V + 255) / 256;
+  float *f = (float *)scratch(ctx, L_X, sizeof(float) * (size_t)vx.V);
+  float *bred = (float *)scratch(ctx, L_T, sizeof(float) * 2 * (size_t)nblk);
+  if (!f || !bred) return HSMC_ERR_NO_MEMORY;
+  hipLaunchKernelGGL(k_voxel_scores, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, g, vx,
+                     far2_of(g), m->net, f, bred, bred + nblk);
+  HSMC_HIP(ctx, hipGetLastError());
+  HSMC_HIP(ctx, hipMemcpyAsync(scores, f, sizeof(float) * vx.V, hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HSMC_OK;
+}
+
+}  // extern "C"

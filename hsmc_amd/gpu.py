@@ -30,7 +30,13 @@ EXPORTS = [
     "hsmc_gpu_any_overlap_scaled", "hsmc_gpu_pair_dist", "hsmc_gpu_rdf_counts",
     "hsmc_gpu_contact_counts", "hsmc_gpu_widom_points", "hsmc_gpu_widom_uniform",
     "hsmc_gpu_set_particles", "hsmc_gpu_set_dr_max",
+    "hsmc_gpu_mlp_create", "hsmc_gpu_mlp_destroy", "hsmc_gpu_mlp_set", "hsmc_gpu_mlp_get",
+    "hsmc_gpu_mlp_forward", "hsmc_gpu_mlp_gradient", "hsmc_gpu_point_descriptors",
+    "hsmc_gpu_mlp_train", "hsmc_gpu_voxel_scores", "hsmc_gpu_widom_learned",
 ]
+
+MLP_IN, MLP_HIDDEN = 12, 64
+MLP_NPARAM = 12 * 64 + 64 + 64 * 64 + 64 + 64 + 1
 
 
 class Replica(C.Structure):
@@ -98,6 +104,19 @@ def load() -> C.CDLL:
         "hsmc_gpu_contact_counts": (i32, [vp, i32, f64, i32, f64, P(i64), P(i64)]),
         "hsmc_gpu_widom_points": (i32, [vp, i32, P(f64), i32, P(C.c_uint8), P(i64)]),
         "hsmc_gpu_widom_uniform": (i32, [vp, i32, u64, u64, i64, i64, P(i64)]),
+        "hsmc_gpu_mlp_create": (i32, [vp, P(vp), u64]),
+        "hsmc_gpu_mlp_destroy": (i32, [vp]),
+        "hsmc_gpu_mlp_set": (i32, [vp, P(C.c_float)]),
+        "hsmc_gpu_mlp_get": (i32, [vp, P(C.c_float)]),
+        "hsmc_gpu_mlp_forward": (i32, [vp, P(C.c_float), i32, P(C.c_float)]),
+        "hsmc_gpu_mlp_gradient": (i32, [vp, P(C.c_float), P(C.c_float), i32, P(C.c_float),
+                                        P(C.c_float)]),
+        "hsmc_gpu_point_descriptors": (i32, [vp, i32, P(f64), i32, P(C.c_float)]),
+        "hsmc_gpu_mlp_train": (i32, [vp, i32, f64, u64, u64, i64, i32, i32, C.c_float, C.c_float,
+                                     P(C.c_float)]),
+        "hsmc_gpu_voxel_scores": (i32, [vp, i32, f64, P(C.c_float), i64, P(i64)]),
+        "hsmc_gpu_widom_learned": (i32, [vp, i32, f64, f64, u64, u64, i64, P(f64), P(i64),
+                                         P(i64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -293,6 +312,86 @@ class Context:
         self._ck(self.L.hsmc_gpu_widom_uniform(self.h, r, int(seed), int(sample), int(first),
                                                int(n), C.byref(hits)), "widom_uniform")
         return hits.value
+
+
+class Mlp:
+    """The learned-proposal network (hsmc_gpu_mlp_*) of one context."""
+
+    def __init__(self, ctx: "Context", seed: int = 1):
+        self.ctx = ctx
+        self.L = ctx.L
+        h = C.c_void_p()
+        ctx._ck(self.L.hsmc_gpu_mlp_create(ctx.h, C.byref(h), int(seed)), "mlp_create")
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.hsmc_gpu_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set(self, params) -> None:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.shape == (MLP_NPARAM,)
+        self.ctx._ck(self.L.hsmc_gpu_mlp_set(self.h, _p(p, C.c_float)), "mlp_set")
+
+    def get(self) -> np.ndarray:
+        p = np.zeros(MLP_NPARAM, dtype=np.float32)
+        self.ctx._ck(self.L.hsmc_gpu_mlp_get(self.h, _p(p, C.c_float)), "mlp_get")
+        return p
+
+    def forward(self, x) -> np.ndarray:
+        xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+        y = np.zeros(len(xx), dtype=np.float32)
+        self.ctx._ck(self.L.hsmc_gpu_mlp_forward(self.h, _p(xx, C.c_float), len(xx),
+                                                 _p(y, C.c_float)), "mlp_forward")
+        return y
+
+    def gradient(self, x, t) -> tuple:
+        xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+        tt = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(MLP_NPARAM, dtype=np.float32)
+        loss = C.c_float(0)
+        self.ctx._ck(self.L.hsmc_gpu_mlp_gradient(self.h, _p(xx, C.c_float), _p(tt, C.c_float),
+                                                  len(xx), C.byref(loss), _p(g, C.c_float)),
+                     "mlp_gradient")
+        return loss.value, g
+
+    def train(self, voxel: float, seed: int, sample: int, n_points: int, epochs: int = 10,
+              batch: int = 4096, lr: float = 1e-3, beta: float = 20.0, r: int = 0) -> float:
+        loss = C.c_float(0)
+        self.ctx._ck(self.L.hsmc_gpu_mlp_train(self.h, r, float(voxel), int(seed), int(sample),
+                                               int(n_points), int(epochs), int(batch),
+                                               float(lr), float(beta), C.byref(loss)), "mlp_train")
+        return loss.value
+
+    def voxel_scores(self, voxel: float, r: int = 0) -> np.ndarray:
+        nv = C.c_int64(0)
+        self.ctx._ck(self.L.hsmc_gpu_voxel_scores(self.h, r, float(voxel), None, 0, C.byref(nv)),
+                     "voxel_scores")
+        out = np.zeros(nv.value, dtype=np.float32)
+        self.ctx._ck(self.L.hsmc_gpu_voxel_scores(self.h, r, float(voxel), _p(out, C.c_float),
+                                                  nv.value, C.byref(nv)), "voxel_scores")
+        return out
+
+    def widom(self, voxel: float, mix: float, seed: int, sample: int, n: int, r: int = 0) -> dict:
+        est = C.c_double(0)
+        done = C.c_int64(0)
+        hits = C.c_int64(0)
+        self.ctx._ck(self.L.hsmc_gpu_widom_learned(self.h, r, float(voxel), float(mix), int(seed),
+                                                   int(sample), int(n), C.byref(est),
+                                                   C.byref(done), C.byref(hits)), "widom_learned")
+        return dict(estimate=est.value, n=done.value, hits=hits.value)
+
+
+def point_descriptors(ctx: "Context", xyz, r: int = 0) -> np.ndarray:
+    q = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros((len(q), MLP_IN), dtype=np.float32)
+    ctx._ck(ctx.L.hsmc_gpu_point_descriptors(ctx.h, r, _p(q, C.c_double), len(q),
+                                             _p(out, C.c_float)), "point_descriptors")
+    return out
 
 
 def lattice(type_: int, n: int, rho: float):

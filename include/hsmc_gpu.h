@@ -176,6 +176,49 @@ int hsmc_gpu_widom_points(hsmc_gpu_ctx *ctx, int replica, const double *xyz, int
 int hsmc_gpu_widom_uniform(hsmc_gpu_ctx *ctx, int replica, uint64_t seed,
                            uint64_t sample, int64_t first, int64_t n, int64_t *hits);
 
+/* ---- learned Widom proposal (no counterpart in the reference, whose
+ * insertions are uniform, compute_widom_chem_pot.c:73-80) -----------------
+ * Voxels of edge ~`voxel` tile the box; a voxel's descriptor is the sorted
+ * distances (minus one diameter) from its centre to its 12 nearest sphere
+ * centres, FP32; a 12->64->64->1 ReLU network with BF16 weights (MFMA,
+ * FP32 accumulation) scores it.  Parameters are packed FP32 in the order
+ * W1[12][64], b1[64], W2[64][64], b2[64], w3[64], b3. */
+#define HSMC_MLP_IN 12
+#define HSMC_MLP_HIDDEN 64
+#define HSMC_MLP_NPARAM (12 * 64 + 64 + 64 * 64 + 64 + 64 + 1)
+
+typedef struct hsmc_gpu_mlp hsmc_gpu_mlp;
+
+/* Network on ctx's device, He-uniform weights from Philox(seed). */
+int hsmc_gpu_mlp_create(hsmc_gpu_ctx *ctx, hsmc_gpu_mlp **out, uint64_t seed);
+int hsmc_gpu_mlp_destroy(hsmc_gpu_mlp *m);
+int hsmc_gpu_mlp_set(hsmc_gpu_mlp *m, const float *params);
+int hsmc_gpu_mlp_get(hsmc_gpu_mlp *m, float *params);
+/* y[k] = network(x[k][0..11]) */
+int hsmc_gpu_mlp_forward(hsmc_gpu_mlp *m, const float *x, int n, float *y);
+/* Mean squared error over n rows and its gradient (packed like params). */
+int hsmc_gpu_mlp_gradient(hsmc_gpu_mlp *m, const float *x, const float *t, int n, float *loss,
+                          float *grad);
+/* Descriptors at given points (m x 12). */
+int hsmc_gpu_point_descriptors(hsmc_gpu_ctx *ctx, int replica, const double *xyz, int m,
+                               float *desc);
+/* Train on n_points uniform insertions of Philox stream (seed, sample):
+ * input the descriptor of the voxel holding the point, target
+ * -beta * max(0, 1 - d_min); Adam, minibatches of `batch`, `epochs` passes. */
+int hsmc_gpu_mlp_train(hsmc_gpu_mlp *m, int replica, double voxel, uint64_t seed,
+                       uint64_t sample, int64_t n_points, int epochs, int batch, float lr,
+                       float beta, float *final_loss);
+/* Scores of all voxels (cap >= *n_voxels; scores may be NULL to query V). */
+int hsmc_gpu_voxel_scores(hsmc_gpu_mlp *m, int replica, double voxel, float *scores,
+                          int64_t cap, int64_t *n_voxels);
+/* Importance-sampled Widom: about n insertions from q = mix * softmax(score)
+ * + (1 - mix) / V, stratified over voxels; *estimate = (1/(V n)) sum over
+ * insertable points of 1/q_v, an unbiased estimate of the insertion
+ * probability; *n_done insertions made, *hits insertable. */
+int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mix,
+                           uint64_t seed, uint64_t sample, int64_t n, double *estimate,
+                           int64_t *n_done, int64_t *hits);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,107 @@
+"""Learned Widom proposal on the GPU: network forward and gradient against the
+FP64 oracle (with the kernels' BF16 rounding points), descriptors against
+brute force, unbiasedness of the importance-weighted estimate, and the gain of
+the trained network over uniform insertion."""
+import math
+
+import numpy as np
+import pytest
+
+from hsmc_amd import gpu
+from oracle import mlp_oracle as M
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx_with(n, rho, sweeps, seed=1):
+    o = O.System(2, n, rho=rho, neigh_dr=1.0, max_part=12, seed=seed)
+    L = o.box()["lx"]
+    conf = o.config()
+    ctx = gpu.Context([dict(n=len(conf), box=(L, L, L), dr_max=0.12, seed=seed)])
+    ctx.upload(conf)
+    ctx.sweeps(sweeps)
+    return ctx, L
+
+
+def test_mlp_forward_matches_oracle():
+    ctx, _ = _ctx_with(4, 0.5, 0)
+    rng = np.random.default_rng(1)
+    p = (rng.standard_normal(M.NPARAM) * 0.2).astype(np.float32)
+    net = gpu.Mlp(ctx)
+    net.set(p)
+    assert np.array_equal(net.get(), p)
+    x = rng.standard_normal((1000, 12)).astype(np.float32)
+    y = net.forward(x)
+    ref, _ = M.forward(p.astype(np.float64), x.astype(np.float64), bf=True)
+    assert np.max(np.abs(y - ref)) <= 2e-3 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_mlp_gradient_matches_oracle():
+    ctx, _ = _ctx_with(4, 0.5, 0)
+    rng = np.random.default_rng(2)
+    p = (rng.standard_normal(M.NPARAM) * 0.2).astype(np.float32)
+    net = gpu.Mlp(ctx)
+    net.set(p)
+    x = rng.standard_normal((777, 12)).astype(np.float32)   # not a multiple of 64
+    t = rng.standard_normal(777).astype(np.float32)
+    loss, g = net.gradient(x, t)
+    rl, rg = M.loss_and_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), bf=True)
+    assert abs(loss - rl) <= 1e-3 * rl
+    scale = np.max(np.abs(rg))
+    for name, a, b in (("W1", M.OW1, M.OB1), ("b1", M.OB1, M.OW2), ("W2", M.OW2, M.OB2),
+                       ("b2", M.OB2, M.OW3), ("w3", M.OW3, M.OB3), ("b3", M.OB3, M.NPARAM)):
+        err = np.max(np.abs(g[a:b] - rg[a:b]))
+        assert err <= 5e-3 * scale, (name, err, scale)
+    # and against the unrounded FP64 gradient, to the BF16 resolution
+    _, g64 = M.loss_and_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64))
+    assert np.linalg.norm(g - g64) <= 0.05 * np.linalg.norm(g64)
+
+
+def test_descriptors_match_brute_force():
+    ctx, L = _ctx_with(6, 0.8, 20)
+    conf = ctx.download()
+    rng = np.random.default_rng(3)
+    pts = rng.random((300, 3)) * L
+    d = gpu.point_descriptors(ctx, pts)
+    ref = M.nearest12(conf, L, pts)
+    # exact up to FP32 rounding whenever the 12 neighbours lie within the
+    # 1.5-sigma descriptor grid (always at this density)
+    assert np.max(np.abs(d - ref)) < 1e-5
+
+
+def _uniform_p(ctx, n_per, samples, seed):
+    h = [ctx.widom_uniform(seed, s, 0, n_per) for s in range(samples)]
+    f = np.array(h) / n_per
+    return f.mean(), f.std(ddof=1) / math.sqrt(samples)
+
+
+def test_learned_estimator_is_unbiased():
+    # rho = 0.5: P_ins ~ 2%, checkable against plain uniform insertion;
+    # an untrained (random) network and a trained one must both agree
+    ctx, L = _ctx_with(8, 0.5, 50)
+    pu, su = _uniform_p(ctx, 200000, 20, 11)
+    for trained in (False, True):
+        net = gpu.Mlp(ctx, seed=5)
+        if trained:
+            net.train(0.25, 99, 0, 200000, epochs=3, batch=4096, lr=2e-3, beta=10.0)
+        est = [net.widom(0.25, 0.9, 21, s, 200000)["estimate"] for s in range(20)]
+        pl, sl = np.mean(est), np.std(est, ddof=1) / math.sqrt(len(est))
+        assert abs(pl - pu) < 4 * math.hypot(su, sl), (trained, pl, pu, su, sl)
+        assert abs(pl / pu - 1) < 0.05
+
+
+def test_trained_network_beats_uniform():
+    # rho = 0.8, N = 4000: a trained proposal lowers the variance per insertion
+    ctx, L = _ctx_with(10, 0.8, 300)
+    n = 400000
+    pu, su = _uniform_p(ctx, n, 16, 31)
+    net = gpu.Mlp(ctx, seed=9)
+    loss = net.train(0.25, 77, 0, 400000, epochs=6, batch=4096, lr=2e-3, beta=20.0)
+    assert math.isfinite(loss)
+    est = [net.widom(0.25, 0.9, 41, s, n)["estimate"] for s in range(16)]
+    pl, sl = np.mean(est), np.std(est, ddof=1) / math.sqrt(len(est))
+    assert abs(pl - pu) < 4 * math.hypot(su, sl), (pl, pu, su, sl)
+    gain = (su / sl) ** 2   # variance ratio at equal insertion counts
+    print(f"rho=0.8 P_ins uniform {pu:.4e}+-{su:.1e} learned {pl:.4e}+-{sl:.1e} gain {gain:.1f}")
+    assert gain > 2.0
