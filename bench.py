@@ -57,9 +57,11 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["n32000", "batched"], default="n32000",
+    ap.add_argument("--workload", choices=["n32000", "batched", "widom"], default="n32000",
                     help="n32000: the headline single system; batched: 256 replicas "
-                         "N = 4 n^3, n in 3..15, rho in [0.30, 0.93]")
+                         "N = 4 n^3, n in 3..15, rho in [0.30, 0.93]; widom: GPU seconds to "
+                         "sigma(beta mu_ex) = 0.01 at N = 108000, rho = 0.90, learned vs uniform")
+    ap.add_argument("--sigma", type=float, default=0.01, help="widom: target sigma(beta mu_ex)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", type=str, default="",
                     help="lanes_per_cell,cells_per_block,sweeps_per_graph (skips the autotuner)")
@@ -86,6 +88,8 @@ def main() -> None:
 
     if args.workload == "batched":
         return batched(args)
+    if args.workload == "widom":
+        return widom(args)
 
     from hsmc_amd import gpu
     conf, box = gpu.lattice(2, 20, 0.85)
@@ -187,6 +191,99 @@ def batched(args) -> None:
                                "in [0.30, 0.93], one Philox seed each", "total_spheres": total,
                    "tuning": tuning,
                    "acceptance": sum(a["accepted"] for a in acc) / max(1, sum(a["attempts"] for a in acc))},
+    }))
+
+
+def _timed(fn):
+    t0 = time.perf_counter()
+    out = fn()
+    return out, time.perf_counter() - t0
+
+
+def widom(args) -> None:
+    """beta mu_ex at rho = 0.90, N = 108000 (fcc 4*30^3, melted): GPU seconds of
+    insertion work until the Monte Carlo error sigma(beta mu_ex) reaches the
+    target, plain uniform insertions against the learned proposal, on the same
+    K configurations (one context each).  Both estimates must agree within 2
+    sigma.  The network is trained once on 10^6 uniform insertions of the
+    first configuration (its time is reported apart)."""
+    import math
+
+    import numpy as np
+
+    from hsmc_amd import analysis as A
+    from hsmc_amd import gpu
+    rho, n_cells, K = 0.90, 30, 4
+    voxel, mix = 0.25, 0.9
+    conf, box = gpu.lattice(2, n_cells, rho)
+    N = len(conf)
+    base = gpu.Context([dict(n=N, box=box, dr_max=0.1, seed=11)])
+    base.upload(conf)
+    base.sweeps(10000)   # the superheated fcc crystal melts (mean square displacement grows linearly)
+    confs = []
+    for k in range(K):
+        base.sweeps(1000)
+        confs.append(base.download())
+    ctxs = []
+    for k in range(K):
+        c = gpu.Context([dict(n=N, box=box, dr_max=0.1, seed=100 + k)])
+        c.upload(confs[k])
+        ctxs.append(c)
+    print(f"[bench] widom: {K} configurations of N={N} at rho={rho}", file=sys.stderr, flush=True)
+
+    # train on 10^6 uniform insertions of configuration 0
+    net0 = gpu.Mlp(ctxs[0], seed=5)
+    loss, t_train = _timed(lambda: net0.train(voxel, 777, 0, 1_000_000, epochs=10, batch=4096,
+                                               lr=2e-3, beta=20.0))
+    params = net0.get()
+    nets = [net0] + [gpu.Mlp(c, seed=5) for c in ctxs[1:]]
+    for m in nets[1:]:
+        m.set(params)
+    print(f"[bench] trained: loss {loss:.4f} in {t_train:.3f} s", file=sys.stderr, flush=True)
+
+    def run(method, n_per, min_rounds=8, max_rounds=4000):
+        est = [[] for _ in range(K)]
+        secs = 0.0
+        n_total = 0
+        rnd = 0
+        while True:
+            for k in range(K):
+                if method == "uniform":
+                    h, dt = _timed(lambda: ctxs[k].widom_uniform(1234 + k, rnd, 0, n_per))
+                    est[k].append(h / n_per)
+                    n_total += n_per
+                else:
+                    r, dt = _timed(lambda: nets[k].widom(voxel, mix, 4321 + k, rnd, n_per))
+                    est[k].append(r["estimate"])
+                    n_total += r["n"]
+                secs += dt
+            rnd += 1
+            if rnd < 2:
+                continue
+            p = np.mean([np.mean(e) for e in est])
+            var = sum(np.var(e, ddof=1) / len(e) for e in est) / K ** 2
+            sig = math.sqrt(var) / p if p > 0 else float("inf")
+            if rnd >= min_rounds and (sig <= args.sigma or rnd >= max_rounds):
+                return dict(beta_mu=-math.log(p), sigma=float(sig), p_insert=float(p), seconds=secs,
+                            insertions=n_total, rounds=rnd,
+                            insertions_per_s=n_total / secs)
+
+    uni = run("uniform", 100_000_000)
+    print(f"[bench] uniform: {uni}", file=sys.stderr, flush=True)
+    lrn = run("learned", 250_000)
+    print(f"[bench] learned: {lrn}", file=sys.stderr, flush=True)
+    z = abs(uni["beta_mu"] - lrn["beta_mu"]) / math.hypot(uni["sigma"], lrn["sigma"])
+    cs = A.carnahan_starling(rho)
+    print(json.dumps({
+        "metric": "GPU seconds to sigma(beta mu_ex)=0.01 at rho=0.90, learned vs uniform Widom",
+        "value": uni["seconds"] / lrn["seconds"], "unit": "x less GPU time (uniform / learned)",
+        "n_gpus": 1, "higher_is_better": True, "vs_baseline": None, "dtype": "f64 (overlap), bf16 (network)",
+        "data": "synthetic (fcc 4*30^3 at rho=0.90 melted by 10^4 sweeps; 4 configurations 10^3 sweeps apart)",
+        "config": {"workload": "N=108000, rho=0.90, voxel 0.25 sigma (198^3), mix 0.9, "
+                               "network 12-64-64-1 trained on 1e6 uniform insertions",
+                   "target_sigma": args.sigma},
+        "uniform": uni, "learned": lrn, "learned_training_seconds": t_train,
+        "agreement_sigmas": z, "carnahan_starling_beta_mu_ex": cs["mu_ex"],
     }))
 
 

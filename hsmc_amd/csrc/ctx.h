@@ -87,6 +87,10 @@ struct hsmc_gpu_ctx {
   double aux_cut = 0.0;
   unsigned long long aux_version = 0;
   int aux_n[3] = {0, 0, 0};
+  // neighbour grids of the learned proposal (descriptor grid, overlap grid)
+  int ngrid_replica[2] = {-1, -1};
+  unsigned long long ngrid_version[2] = {0, 0};
+  double ngrid_cut[2] = {0.0, 0.0};
   // scratch buffers of the observables, grown on demand
   static constexpr int kScratch = 40;
   void *scratch[kScratch] = {};

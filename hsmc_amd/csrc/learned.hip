@@ -614,6 +614,12 @@ struct hsmc_gpu_mlp {
   MlpDev *net = nullptr;
   float *loss = nullptr;
   long step = 0;
+  // voxel scores of the last configuration scored (replica, version, step,
+  // voxel) with their max and sum exp(f - max)
+  int sc_replica = -1;
+  unsigned long long sc_version = 0;
+  long sc_step = -1;
+  double sc_voxel = 0.0, sc_fmax = 0.0, sc_sumexp = 0.0;
 };
 
 namespace {
@@ -655,7 +661,7 @@ int grid_cells_for2(double L, double cut) {
 // rows / scores 30, targets / block partials 31, counts 32, offsets 33,
 // sums 34, insertion list 35, scan partials 36..37.
 enum { L_GRID_A = 24, L_GRID_B = 27, L_X = 30, L_T = 31, L_CNT = 32, L_OFF = 33, L_ACC = 34,
-       L_LIST = 35, L_BSUM = 36, L_BOFF = 37 };
+       L_LIST = 35, L_BSUM = 36, L_BOFF = 37, L_F = 38, L_FRED = 39 };
 
 int build_ngrid(hsmc_gpu_ctx *ctx, int replica, double cut, int slot0, NGrid &g) {
   const RepHost &h = ctx->reps[replica];
@@ -663,6 +669,9 @@ int build_ngrid(hsmc_gpu_ctx *ctx, int replica, double cut, int slot0, NGrid &g)
   const int nx = grid_cells_for2(h.p.lx, cut), ny = grid_cells_for2(h.p.ly, cut),
             nz = grid_cells_for2(h.p.lz, cut);
   const long C = (long)nx * ny * nz;
+  const int which = slot0 == L_GRID_A ? 0 : 1;
+  const bool cached = ctx->ngrid_replica[which] == replica &&
+                      ctx->ngrid_version[which] == ctx->version && ctx->ngrid_cut[which] == cut;
   int *cnt = (int *)scratch(ctx, slot0 + 0, sizeof(int) * 2 * (C + 1) + sizeof(int) * N);
   double *pos = (double *)scratch(ctx, slot0 + 1, sizeof(double) * 3 * N);
   int *start = (int *)scratch(ctx, slot0 + 2, sizeof(int) * (C + 1));
@@ -670,20 +679,24 @@ int build_ngrid(hsmc_gpu_ctx *ctx, int replica, double cut, int slot0, NGrid &g)
   int *fill = cnt + (C + 1);
   int *cellof = fill + (C + 1);
   const double *x = ctx->d_x + h.poff, *y = ctx->d_y + h.poff, *z = ctx->d_z + h.poff;
-  HSMC_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(int) * 2 * (C + 1), ctx->stream));
   const double sx = h.p.lx / nx, sy = h.p.ly / ny, sz = h.p.lz / nz;
-  hipLaunchKernelGGL(k_grid_count2, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, x, y, z, N,
-                     nx, ny, nz, sx, sy, sz, cnt, cellof);
-  hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, ctx->stream, cnt, (int)C, start);
-  hipLaunchKernelGGL(k_grid_scatter2, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, x, y, z,
-                     N, cellof, start, fill, pos, pos + N, pos + 2 * (size_t)N);
-  HSMC_HIP(ctx, hipGetLastError());
   g.nx = nx; g.ny = ny; g.nz = nz;
   g.sx = sx; g.sy = sy; g.sz = sz;
   g.lx = h.p.lx; g.ly = h.p.ly; g.lz = h.p.lz;
   g.hx = h.p.lx / 2.0; g.hy = h.p.ly / 2.0; g.hz = h.p.lz / 2.0;
   g.start = start;
   g.px = pos; g.py = pos + N; g.pz = pos + 2 * (size_t)N;
+  if (cached) return HSMC_OK;
+  HSMC_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(int) * 2 * (C + 1), ctx->stream));
+  hipLaunchKernelGGL(k_grid_count2, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, x, y, z, N,
+                     nx, ny, nz, sx, sy, sz, cnt, cellof);
+  hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, ctx->stream, cnt, (int)C, start);
+  hipLaunchKernelGGL(k_grid_scatter2, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, x, y, z,
+                     N, cellof, start, fill, pos, pos + N, pos + 2 * (size_t)N);
+  HSMC_HIP(ctx, hipGetLastError());
+  ctx->ngrid_replica[which] = replica;
+  ctx->ngrid_version[which] = ctx->version;
+  ctx->ngrid_cut[which] = cut;
   return HSMC_OK;
 }
 
@@ -779,6 +792,7 @@ int hsmc_gpu_mlp_set(hsmc_gpu_mlp *m, const float *params) {
   HSMC_HIP(ctx, hipGetLastError());
   HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   m->step = 0;
+  m->sc_replica = -1;
   return HSMC_OK;
 }
 
@@ -902,22 +916,32 @@ int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mi
   const Vox vx = make_vox(ctx->reps[replica], voxel);
   if (vx.V > (1L << 31) - 1) return HSMC_ERR_NOT_SUPPORTED;
   const long nblk = (vx.V + 255) / 256;
-  float *f = (float *)scratch(ctx, L_X, sizeof(float) * (size_t)vx.V);
-  float *bred = (float *)scratch(ctx, L_T, sizeof(float) * 2 * (size_t)nblk);
+  float *f = (float *)scratch(ctx, L_F, sizeof(float) * (size_t)vx.V);
+  float *bred = (float *)scratch(ctx, L_FRED, sizeof(float) * 2 * (size_t)nblk);
   int *cnt = (int *)scratch(ctx, L_CNT, sizeof(int) * (size_t)vx.V);
   int *off = (int *)scratch(ctx, L_OFF, sizeof(int) * (size_t)(vx.V + 1));
   double *acc = (double *)scratch(ctx, L_ACC, sizeof(double) * 2);
   if (!f || !bred || !cnt || !off || !acc) return HSMC_ERR_NO_MEMORY;
-  hipLaunchKernelGGL(k_voxel_scores, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, g, vx,
-                     far2_of(g), m->net, f, bred, bred + nblk);
-  HSMC_HIP(ctx, hipGetLastError());
-  std::vector<float> hb(2 * nblk);
-  HSMC_HIP(ctx, hipMemcpyAsync(hb.data(), bred, sizeof(float) * 2 * nblk, hipMemcpyDeviceToHost, ctx->stream));
-  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  double fmax = -INFINITY;
-  for (long b = 0; b < nblk; ++b) fmax = std::max(fmax, (double)hb[b]);
-  double sumexp = 0.0;
-  for (long b = 0; b < nblk; ++b) sumexp += (double)hb[nblk + b] * exp((double)hb[b] - fmax);
+  if (!(m->sc_replica == replica && m->sc_version == ctx->version && m->sc_step == m->step &&
+        m->sc_voxel == voxel)) {
+    hipLaunchKernelGGL(k_voxel_scores, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, g, vx,
+                       far2_of(g), m->net, f, bred, bred + nblk);
+    HSMC_HIP(ctx, hipGetLastError());
+    std::vector<float> hb(2 * nblk);
+    HSMC_HIP(ctx, hipMemcpyAsync(hb.data(), bred, sizeof(float) * 2 * nblk, hipMemcpyDeviceToHost, ctx->stream));
+    HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double fm = -INFINITY;
+    for (long b = 0; b < nblk; ++b) fm = std::max(fm, (double)hb[b]);
+    double se = 0.0;
+    for (long b = 0; b < nblk; ++b) se += (double)hb[nblk + b] * exp((double)hb[b] - fm);
+    m->sc_replica = replica;
+    m->sc_version = ctx->version;
+    m->sc_step = m->step;
+    m->sc_voxel = voxel;
+    m->sc_fmax = fm;
+    m->sc_sumexp = se;
+  }
+  const double fmax = m->sc_fmax, sumexp = m->sc_sumexp;
   hipLaunchKernelGGL(k_allocate, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, f, vx.V, fmax,
                      sumexp, mix, (double)n, (uint32_t)seed, (uint32_t)(seed >> 32),
                      (unsigned long long)sample, cnt);
@@ -957,9 +981,10 @@ int hsmc_gpu_voxel_scores(hsmc_gpu_mlp *m, int replica, double voxel, float *sco
   int st = build_ngrid(ctx, replica, kDescCut, L_GRID_A, g);
   if (st) return st;
   const long nblk = (vx.V + 255) / 256;
-  float *f = (float *)scratch(ctx, L_X, sizeof(float) * (size_t)vx.V);
-  float *bred = (float *)scratch(ctx, L_T, sizeof(float) * 2 * (size_t)nblk);
+  float *f = (float *)scratch(ctx, L_F, sizeof(float) * (size_t)vx.V);
+  float *bred = (float *)scratch(ctx, L_FRED, sizeof(float) * 2 * (size_t)nblk);
   if (!f || !bred) return HSMC_ERR_NO_MEMORY;
+  m->sc_replica = -1;
   hipLaunchKernelGGL(k_voxel_scores, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, g, vx,
                      far2_of(g), m->net, f, bred, bred + nblk);
   HSMC_HIP(ctx, hipGetLastError());
