@@ -112,7 +112,7 @@ def read_rdf_file(path: str) -> list:
 
 def read_columns(path: str) -> np.ndarray:
     """chem_pot.dat / density.dat: '#' comment lines, then numeric rows."""
-    return np.atleast_2d(np.loadtxt(path, comments="#"))
+    return np.loadtxt(path, comments="#", ndmin=2)
 
 
 def read_config_file(path: str) -> list:

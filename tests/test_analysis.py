@@ -75,3 +75,13 @@ def test_widom_estimators():
     w = A.widom_mu(rows, 100)
     assert abs(w["mu_reference_estimator"] - 0.5 * (math.log(2) + math.log(4))) < 1e-12
     assert abs(w["mu_ex"] + math.log(0.25)) < 1e-12
+
+
+def test_read_columns_single_and_multi_column(tmp_path):
+    """density.dat holds one column, chem_pot.dat two; both read as (rows, cols)."""
+    (tmp_path / "d.dat").write_text("####\n# Density\n####\n7.5e-01\n7.6e-01\n7.4e-01\n")
+    d = A.read_columns(str(tmp_path / "d.dat"))
+    assert d.shape == (3, 1) and d[1, 0] == 0.76
+    (tmp_path / "m.dat").write_text("# mu, frac\n1.0 0.5\n")
+    m = A.read_columns(str(tmp_path / "m.dat"))
+    assert m.shape == (1, 2) and m[0, 1] == 0.5
