@@ -445,6 +445,19 @@ int hsmc_gpu_reset_counters(hsmc_gpu_ctx *ctx) {
   return HSMC_OK;
 }
 
+int hsmc_gpu_set_sweep_counter(hsmc_gpu_ctx *ctx, uint64_t sweep) {
+  if (!ctx) return HSMC_ERR_BAD_ARG;
+  unsigned long long now = 0;
+  HSMC_HIP(ctx, hipMemcpyAsync(&now, ctx->d_sweep, sizeof(now), hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const unsigned long long want = (unsigned long long)sweep;
+  HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_sweep, &want, sizeof(want), hipMemcpyHostToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  // attempts are N * (sweep - sweep_at_reset): move the origin with the counter
+  ctx->sweep_at_reset += want - now;
+  return HSMC_OK;
+}
+
 int hsmc_gpu_set_tuning(hsmc_gpu_ctx *ctx, const hsmc_gpu_tuning *t) {
   if (!ctx || !t || !tuning_ok(ctx, *t)) return HSMC_ERR_BAD_ARG;
   ctx->tune = *t;

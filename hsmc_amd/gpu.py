@@ -24,7 +24,8 @@ EXPORTS = [
     "hsmc_gpu_create", "hsmc_gpu_destroy", "hsmc_gpu_strerror", "hsmc_gpu_last_error",
     "hsmc_gpu_sync", "hsmc_gpu_upload", "hsmc_gpu_download", "hsmc_gpu_set_box",
     "hsmc_gpu_scale", "hsmc_gpu_grid", "hsmc_gpu_sweeps", "hsmc_gpu_time_sweeps",
-    "hsmc_gpu_counters", "hsmc_gpu_reset_counters", "hsmc_gpu_set_tuning",
+    "hsmc_gpu_counters", "hsmc_gpu_reset_counters", "hsmc_gpu_set_sweep_counter",
+    "hsmc_gpu_set_tuning",
     "hsmc_gpu_get_tuning", "hsmc_gpu_autotune", "hsmc_gpu_time_phase_kernel",
     "hsmc_gpu_cell_list", "hsmc_gpu_overlap_batch", "hsmc_gpu_overlap_scaled",
     "hsmc_gpu_any_overlap_scaled", "hsmc_gpu_pair_dist", "hsmc_gpu_rdf_counts",
@@ -91,6 +92,7 @@ def load() -> C.CDLL:
         "hsmc_gpu_time_sweeps": (i32, [vp, i32, P(C.c_float)]),
         "hsmc_gpu_counters": (i32, [vp, i32, P(u64)]),
         "hsmc_gpu_reset_counters": (i32, [vp]),
+        "hsmc_gpu_set_sweep_counter": (i32, [vp, u64]),
         "hsmc_gpu_set_tuning": (i32, [vp, P(Tuning)]),
         "hsmc_gpu_get_tuning": (i32, [vp, P(Tuning)]),
         "hsmc_gpu_autotune": (i32, [vp, i32, P(Tuning)]),
@@ -228,6 +230,9 @@ class Context:
 
     def reset_counters(self) -> None:
         self._ck(self.L.hsmc_gpu_reset_counters(self.h), "reset_counters")
+
+    def set_sweep_counter(self, sweep: int) -> None:
+        self._ck(self.L.hsmc_gpu_set_sweep_counter(self.h, sweep), "set_sweep_counter")
 
     def set_tuning(self, lanes_per_cell: int, cells_per_block: int, sweeps_per_graph: int) -> None:
         t = Tuning(lanes_per_cell, cells_per_block, sweeps_per_graph, 0)

@@ -16,6 +16,8 @@ void host_gpu_box_sync(void);
 void host_count_sweep_moves(uint64_t attempts, uint64_t accepted);
 /* 64-bit Philox key of the GPU random streams */
 uint64_t host_philox_seed(void);
+uint64_t host_sweep_counter(void);
+void host_set_sweep_counter(uint64_t sweep);
 /* seed the MT19937 stream */
 void rng_seed(unsigned long s);
 

@@ -326,10 +326,12 @@ void write_config(int sweep) {
   }
 }
 
-/* Restart file: the reference's fields in its order (io_config.c:53-69):
- * dr_max, dv_max, box_info, p_info, N x 4 doubles, MT19937 state; then an
- * optional trailer with the GPU stream position (ignored by a reader that
- * stops after the MT state). */
+/* Restart file: the reference's fields in its order and layout
+ * (io_config.c:53-69, sim_info.c:220-249): dr_max, dv_max, box_info,
+ * p_info, N x 4 doubles, MT19937 state (GSL's mt19937 state struct); then a
+ * trailer with the sweep-stream position of the GPU sweep (an 8-byte tag and
+ * the counter), which a reader that stops after the MT state ignores. */
+static const uint64_t kRestartTag = 0x3130504753434d48ull;   /* "HMCSGP01" */
 void write_restart(int sweep) {
   if (restart_output_check) {
     if ((double)(G_IN.sweep_stat + G_IN.sweep_eq) / (G_IN.restart_write) > 100000)
@@ -352,6 +354,8 @@ void write_restart(int sweep) {
   part_info_write(fid);
   part_conf_write(fid);
   rng_write(fid);
+  uint64_t trailer[2] = {kRestartTag, host_sweep_counter()};
+  fwrite(trailer, sizeof(uint64_t), 2, fid);
   fclose(fid);
 }
 
@@ -371,6 +375,9 @@ void read_restart(char *restart_file) {
   part_alloc();
   part_conf_read(fid);
   rng_read(fid);
+  uint64_t trailer[2];
+  if (fread(trailer, sizeof(uint64_t), 2, fid) == 2 && trailer[0] == kRestartTag)
+    host_set_sweep_counter(trailer[1]);
   fclose(fid);
   box_info b = sim_box_info_get();
   p_info pi = part_info_get();

@@ -17,6 +17,8 @@
 #include "hsmc_gpu.h"
 #include "host_internal.h"
 
+static uint64_t pending_sweep_counter = 0;
+
 /* ---- box and particles (sim_info.c) ---- */
 static box_info sim_box;
 static p_info part_inf;
@@ -311,6 +313,24 @@ static void ensure_gpu(void) {
   rep.seed = host_philox_seed();
   host_gpu_check(hsmc_gpu_create(&gpu, G_IN.gpu_device, 1, &rep), "hsmc_gpu_create");
   dev_behind = 1;
+  if (pending_sweep_counter) {
+    host_gpu_check(hsmc_gpu_set_sweep_counter(gpu, pending_sweep_counter), "set_sweep_counter");
+    pending_sweep_counter = 0;
+  }
+}
+
+/* Sweep-stream position for restart files: read from the context, or set
+ * (now, or when the context is created). */
+uint64_t host_sweep_counter(void) {
+  if (!gpu) return pending_sweep_counter;
+  uint64_t c[4];
+  host_gpu_check(hsmc_gpu_counters(gpu, 0, c), "counters");
+  return c[3];
+}
+
+void host_set_sweep_counter(uint64_t sweep) {
+  if (gpu) host_gpu_check(hsmc_gpu_set_sweep_counter(gpu, sweep), "set_sweep_counter");
+  else pending_sweep_counter = sweep;
 }
 
 /* Box of the GPU context follows sim_box (after restart or volume moves). */

@@ -115,6 +115,11 @@ int hsmc_gpu_time_sweeps(hsmc_gpu_ctx *ctx, int n_sweeps, float *ms);
  * sweeps done (moves.c:22-24 part_moves / acc_part_moves, kept 64-bit). */
 int hsmc_gpu_counters(hsmc_gpu_ctx *ctx, int replica, uint64_t out[4]);
 int hsmc_gpu_reset_counters(hsmc_gpu_ctx *ctx);
+/* Position of the sweep streams (the sweep counter in the Philox counters),
+ * as counters() out[3] reports it; restored from a restart file so a resumed
+ * run continues the same streams (the restart of io_config.c:28-130 keeps
+ * the MT19937 state for the same purpose).  Attempt counts are unchanged. */
+int hsmc_gpu_set_sweep_counter(hsmc_gpu_ctx *ctx, uint64_t sweep);
 int hsmc_gpu_set_tuning(hsmc_gpu_ctx *ctx, const hsmc_gpu_tuning *t);
 int hsmc_gpu_get_tuning(const hsmc_gpu_ctx *ctx, hsmc_gpu_tuning *t);
 /* Time the candidate launch shapes on this context's own replicas (their

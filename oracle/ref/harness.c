@@ -1,8 +1,9 @@
 /*
  * harness.c -- drives fedluc/HSMC's own hot-path functions, compiled from
  * the reference sources where they lie (see oracle/ref/Makefile), to write
- * golden vectors.  TEST INFRASTRUCTURE ONLY: it runs in the build container,
- * never on the GPU box, and only its numeric outputs are committed (under
+ * golden vectors.  TEST INFRASTRUCTURE ONLY: it runs where the reference
+ * sources are present, never beside the GPU runs, and only its numeric
+ * outputs are committed (under
  * tests/golden/).  No reference source is copied; the module-static state of
  * three observable files is reached by compiling them into this translation
  * unit with #include of their path under the reference tree (the -I flag in

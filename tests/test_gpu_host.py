@@ -327,3 +327,37 @@ press_thermo 0.0001 0.002 5
     assert abs(m_gpu - m_cpu) < 4.0 * se, (rho_gpu, rho_cpu, se)
     # Carnahan-Starling density at beta P = 3 is 0.6348 (N = 256 sits a little above)
     assert abs(m_gpu - 0.6348) < 0.012, rho_gpu
+
+
+def test_checkerboard_restart_continues_the_same_run(tmp_path):
+    """A checkerboard run resumed from restart_5.bin (configuration, MT
+    state and the sweep-stream position in the trailer) ends in the same
+    configuration as the uninterrupted run: 8 sweeps either way."""
+    common = """cells_x 4
+cells_y 4
+cells_z 4
+type 2
+neigh_list 1.05 10
+opt 0 1000 10 0.5 0.5
+seed 17
+out 100
+config_write 1 1
+"""
+    a = tmp_path / "a"
+    a.mkdir()
+    run_cli(a, "rho 0.7\ndr_max 0.12\nsweep_eq 8\nsweep_stat 1\nrestart_write 5\n" + common)
+    b = tmp_path / "b"
+    b.mkdir()
+    (b / "restart_5.bin").write_bytes((a / "restart_5.bin").read_bytes())
+    run_cli(b, "restart_read 1 restart_5.bin\nsweep_eq 3\nsweep_stat 1\n" + common)
+    ca = A.read_config_file(str(a / "config_000000.dat.gz"))[-1]
+    cb = A.read_config_file(str(b / "config_000000.dat.gz"))[-1]
+    assert ca[1] == cb[1]
+    assert np.array_equal(ca[2], cb[2])
+    # the resumed run moved the spheres: restart_5.bin holds dr_max, dv_max
+    # (16 B), box_info (64 B), p_info (8 B), then N x 4 doubles
+    raw = (a / "restart_5.bin").read_bytes()
+    n = int(np.frombuffer(raw, dtype=np.int32, count=1, offset=80)[0])
+    conf5 = np.frombuffer(raw, dtype=np.float64, count=4 * n, offset=88).reshape(n, 4)
+    moved = np.abs(np.round(conf5[:, 1:], 8) - cb[2][:, 1:]).max(axis=1) > 0
+    assert n == len(cb[2]) and moved.mean() > 0.5
