@@ -34,6 +34,7 @@ struct RepDev {
   double lx, ly, lz;     // box
   double hx, hy, hz;     // half box (lx/2.0 ...)
   double sx, sy, sz;     // sweep-grid cell side (lx/gx ...)
+  double ix, iy, iz;     // 1.0 / sx ...: the sweep grid's cell index is (int)(t * ix)
   double dr_max;         // displacement width
   int N, gx, gy, gz;     // particles, sweep grid
   int poff, coff, aoff;  // offsets into particle, cell and active-cell arrays
@@ -105,6 +106,8 @@ int hip_fail(hsmc_gpu_ctx *ctx, hipError_t e, const char *what);
 void *scratch(hsmc_gpu_ctx *ctx, int slot, size_t bytes);
 // (Re)build replica offsets, the sweep grids, and the device cell arrays.
 int layout(hsmc_gpu_ctx *ctx);
+// LDS bytes of one sublattice-phase workgroup of cpb cells
+size_t phase_lds_bytes(int cap, int lcap, int cpb);
 // Sweep-grid cells per axis for a box edge: largest even g with L/g > 1.
 int sweep_grid_cells(double L);
 // Read and clear the kernels' sticky error bits; translate to a status.

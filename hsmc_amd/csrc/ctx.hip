@@ -56,20 +56,22 @@ void drop_graph(hsmc_gpu_ctx *ctx) {
 }
 
 int fold_counters(hsmc_gpu_ctx *ctx) {
-  if (!ctx->d_cacc || ctx->Ctot == 0) return HSMC_OK;
-  std::vector<unsigned int> a(ctx->Ctot), l(ctx->Ctot);
-  HSMC_HIP(ctx, hipMemcpyAsync(a.data(), ctx->d_cacc, sizeof(unsigned int) * ctx->Ctot,
+  // the counters are indexed by active-cell index (Atot entries, replica r
+  // owning [aoff, aoff + cells/8))
+  if (!ctx->d_cacc || ctx->Atot == 0) return HSMC_OK;
+  std::vector<unsigned int> a(ctx->Atot), l(ctx->Atot);
+  HSMC_HIP(ctx, hipMemcpyAsync(a.data(), ctx->d_cacc, sizeof(unsigned int) * ctx->Atot,
                                hipMemcpyDeviceToHost, ctx->stream));
-  HSMC_HIP(ctx, hipMemcpyAsync(l.data(), ctx->d_cleft, sizeof(unsigned int) * ctx->Ctot,
+  HSMC_HIP(ctx, hipMemcpyAsync(l.data(), ctx->d_cleft, sizeof(unsigned int) * ctx->Atot,
                                hipMemcpyDeviceToHost, ctx->stream));
-  HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cacc, 0, sizeof(unsigned int) * ctx->Ctot, ctx->stream));
-  HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cleft, 0, sizeof(unsigned int) * ctx->Ctot, ctx->stream));
+  HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cacc, 0, sizeof(unsigned int) * ctx->Atot, ctx->stream));
+  HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cleft, 0, sizeof(unsigned int) * ctx->Atot, ctx->stream));
   HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   for (int r = 0; r < ctx->R; ++r) {
     const RepHost &h = ctx->reps[r];
-    const long n = (long)h.gx * h.gy * h.gz;
+    const long n = (long)h.gx * h.gy * h.gz / 8;
     unsigned long long sa = 0, sl = 0;
-    for (long c = h.coff; c < h.coff + n; ++c) {
+    for (long c = h.aoff; c < h.aoff + n; ++c) {
       sa += a[c];
       sl += l[c];
     }
@@ -107,6 +109,7 @@ int layout(hsmc_gpu_ctx *ctx) {
     d.lx = p.lx; d.ly = p.ly; d.lz = p.lz;
     d.hx = p.lx / 2.0; d.hy = p.ly / 2.0; d.hz = p.lz / 2.0;
     d.sx = p.lx / h.gx; d.sy = p.ly / h.gy; d.sz = p.lz / h.gz;
+    d.ix = 1.0 / d.sx; d.iy = 1.0 / d.sy; d.iz = 1.0 / d.sz;
     // a cube of side < 2/sqrt(3) holds at most 8 points 1 apart
     if (d.sx >= 1.1547 || d.sy >= 1.1547 || d.sz >= 1.1547) big = true;
     d.dr_max = p.dr_max;
@@ -152,6 +155,7 @@ int layout(hsmc_gpu_ctx *ctx) {
   ctx->Ctot = coff;
   ctx->Atot = aoff;
   const int cap = big ? 16 : 8;
+  if (coff * cap >= (1L << 31)) return HSMC_ERR_NOT_SUPPORTED;  // slot indices are int
   const size_t need = (size_t)coff;
   if (need > ctx->cell_alloc || cap != ctx->cap || !ctx->d_cnt) {
     (void)hipStreamSynchronize(ctx->stream);
@@ -174,9 +178,8 @@ int layout(hsmc_gpu_ctx *ctx) {
   }
   // keep the phase kernel's LDS staging within 64 KiB per workgroup
   {
-    const size_t per_cell = 3 * (size_t)ctx->lcap * sizeof(double) + 3 * (size_t)ctx->cap * sizeof(int);
     while (ctx->tune.cells_per_block > 1 &&
-           per_cell * (size_t)ctx->tune.cells_per_block > 64 * 1024)
+           phase_lds_bytes(ctx->cap, ctx->lcap, ctx->tune.cells_per_block) > 64 * 1024)
       ctx->tune.cells_per_block >>= 1;
   }
   HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cnt, 0, sizeof(int) * 2 * ctx->cell_alloc, ctx->stream));

@@ -89,13 +89,15 @@ __host__ __device__ __forceinline__ double wrap_once(double x, double L) {
 }
 
 // Cell of coordinate x on a grid of g cells of side `side` whose origin is
-// shifted by `shift` in [0, side): t = x - shift (wrapped), floor(t/side),
-// clamped to g-1 for t == L.
+// shifted by `shift` in [0, side): t = x - shift (wrapped), (int)(t * (1/side)),
+// clamped to g-1 at t == L.  Multiplying by the rounded reciprocal moves each
+// cell edge by a few ulp at most, so every cell stays wider than the grid's
+// minimum side of 1 + 1e-9 diameters.
 __host__ __device__ __forceinline__ int shifted_cell(double x, double shift, double L,
-                                                     double side, int g) {
+                                                     double inv_side, int g) {
   double t = x - shift;
   if (t < 0.0) t += L;
-  int c = (int)(t / side);
+  int c = (int)(t * inv_side);
   if (c >= g) c = g - 1;
   return c;
 }

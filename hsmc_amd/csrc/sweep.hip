@@ -29,6 +29,12 @@ using namespace hsmc;
 
 namespace {
 
+// Vector store at agent scope: written through the XCD's L2 (the next
+// phase's workgroups read it from other XCDs).
+__device__ __forceinline__ void store_through(double *p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -79,9 +85,9 @@ k_rebin(const RepDev *__restrict__ reps, const int *__restrict__ prep, int P, in
   double shx, shy, shz;
   sweep_shift(rp, sweep, shx, shy, shz);
   const double px = x[i], py = y[i], pz = z[i];
-  const int cx = shifted_cell(px, shx, rp.lx, rp.sx, rp.gx);
-  const int cy = shifted_cell(py, shy, rp.ly, rp.sy, rp.gy);
-  const int cz = shifted_cell(pz, shz, rp.lz, rp.sz, rp.gz);
+  const int cx = shifted_cell(px, shx, rp.lx, rp.ix, rp.gx);
+  const int cy = shifted_cell(py, shy, rp.ly, rp.iy, rp.gy);
+  const int cz = shifted_cell(pz, shz, rp.lz, rp.iz, rp.gz);
   const int c = rp.coff + (cx * rp.gy + cy) * rp.gz + cz;
   const int slot = atomicAdd(&cnt[(size_t)buf * C + c], 1);
   if (slot < cap) {
@@ -95,6 +101,11 @@ k_rebin(const RepDev *__restrict__ reps, const int *__restrict__ prep, int P, in
   }
 }
 
+// A lane's copies of its share of the stencil list (entries lane,
+// lane + LPC, ...): a vector value, so variable indexing stays in registers.
+template <int M>
+using dvec = double __attribute__((ext_vector_type(M)));
+
 template <int CAP, int LPC, int LCAP>
 __global__ void __launch_bounds__(256)
 k_phase(const RepDev *__restrict__ reps, const int *__restrict__ arep, int R, int A, int phase,
@@ -103,19 +114,18 @@ k_phase(const RepDev *__restrict__ reps, const int *__restrict__ arep, int R, in
         const int *__restrict__ spid, double *__restrict__ sx, double *__restrict__ sy,
         double *__restrict__ sz, unsigned int *__restrict__ cacc,
         unsigned int *__restrict__ cleft, unsigned int *__restrict__ err) {
-  // LDS per active cell: positions of the spheres of its 27 neighbour cells
-  // (at most LCAP, a bound the host derives from the cell side), and slot,
-  // particle id and id order of its own occupants
-  extern __shared__ double smem[];
+  // The stencil list (the spheres of the 27 neighbour cells, at most LCAP, a
+  // bound the host derives from the cell side) lives in registers: entry e
+  // is held by lane e % LPC as its copy e / LPC.  LDS holds the slot of each
+  // entry and the particle id and id order of the active cell's occupants.
+  constexpr int M = (LCAP + LPC - 1) / LPC;
+  extern __shared__ int smem_i[];
   const int cpb = blockDim.x / LPC;
   const int lane = threadIdx.x & (LPC - 1);
   const int sub = threadIdx.x / LPC;
-  double *Lx = smem + (size_t)sub * (3 * LCAP);
-  double *Ly = Lx + LCAP;
-  double *Lz = Ly + LCAP;
-  int *Ss = reinterpret_cast<int *>(smem + (size_t)cpb * 3 * LCAP) + (size_t)sub * (3 * CAP);
-  int *Sp = Ss + CAP;
-  int *Lo = Sp + CAP;
+  int *Lm = smem_i + (size_t)sub * (LCAP + 2 * CAP);  // slot of each list entry
+  int *Sp = Lm + LCAP;                                 // ids of the active cell's occupants
+  int *Lo = Sp + CAP;                                  // occupants in ascending id order
 
   const int a = blockIdx.x * cpb + sub;
   if (a >= A) return;  // the whole lane group leaves together
@@ -188,20 +198,27 @@ k_phase(const RepDev *__restrict__ reps, const int *__restrict__ arep, int R, in
     my_dz = (u32_unit_closed(w[3]) - 0.5) * rp.dr_max;
   }
 
-  // stage the 27 cells' spheres in LDS (FP64 SoA)
-  for (int q = 0; q < nj; ++q) {
-    const int e = off + q;
-    const size_t s = (size_t)ncell * CAP + q;
-    Lx[e] = sx[s];
-    Ly[e] = sy[s];
-    Lz[e] = sz[s];
-    if (lane == 13) {
-      Ss[q] = (int)s;
-      Sp[q] = spid[s];
+  // stage the stencil list: the slot of every entry into LDS, then each
+  // lane loads its entries' FP64 positions into registers, all loads of the
+  // work item in flight together
+  for (int q = 0; q < nj; ++q) Lm[off + q] = ncell * CAP + q;
+  wave_lds_sync();
+  dvec<M> px, py, pz;
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    const int q = lane + j * LPC;
+    px[j] = py[j] = pz[j] = 0.0;
+    if (q < T) {
+      const int s = Lm[q];
+      px[j] = sx[s];
+      py[j] = sy[s];
+      pz[j] = sz[s];
     }
   }
+  if (lane < nc) Sp[lane] = spid[Lm[oc + lane]];
   wave_lds_sync();
-  // occupants of the active cell in ascending particle id
+  // occupants of the active cell in ascending particle id; attempt t moves
+  // list entry oc + Lo[k_t]
   if (lane < nc) {
     const int me = Sp[lane];
     int rank = 0;
@@ -209,63 +226,72 @@ k_phase(const RepDev *__restrict__ reps, const int *__restrict__ arep, int R, in
     Lo[rank] = lane;
   }
   wave_lds_sync();
+  const int my_e = lane < nc ? oc + Lo[my_k] : 0;
 
   unsigned int n_acc = 0, n_left = 0;
   for (int t = 0; t < nc; ++t) {
-    const int kk = __shfl((int)my_k, t, LPC);
+    const int e = __shfl(my_e, t, LPC);
     const double ddx = __shfl(my_dx, t, LPC);
     const double ddy = __shfl(my_dy, t, LPC);
     const double ddz = __shfl(my_dz, t, LPC);
-    const int e = oc + Lo[kk];
-    double nx = Lx[e] + ddx;
-    double ny = Ly[e] + ddy;
-    double nz = Lz[e] + ddz;
+    const int owner = e % LPC, je = e / LPC;
+    const double ex = __shfl((double)px[je], owner, LPC);
+    const double ey = __shfl((double)py[je], owner, LPC);
+    const double ez = __shfl((double)pz[je], owner, LPC);
+    double nx = ex + ddx;
+    double ny = ey + ddy;
+    double nz = ez + ddz;
     nx = wrap_once(nx, rp.lx);
     ny = wrap_once(ny, rp.ly);
     nz = wrap_once(nz, rp.lz);
-    if (shifted_cell(nx, shx, rp.lx, rp.sx, rp.gx) != cx ||
-        shifted_cell(ny, shy, rp.ly, rp.sy, rp.gy) != cy ||
-        shifted_cell(nz, shz, rp.lz, rp.sz, rp.gz) != cz) {
+    if (shifted_cell(nx, shx, rp.lx, rp.ix, rp.gx) != cx ||
+        shifted_cell(ny, shy, rp.ly, rp.iy, rp.gy) != cy ||
+        shifted_cell(nz, shz, rp.lz, rp.iz, rp.gz) != cz) {
       ++n_left;
       continue;
     }
     bool ov = false;
-    for (int q = lane; q < T; q += LPC) {
-      if (q != e &&
-          dist2(nx, ny, nz, Lx[q], Ly[q], Lz[q], rp.lx, rp.ly, rp.lz, rp.hx, rp.hy, rp.hz) < 1.0)
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      const int q = lane + j * LPC;
+      if (q < T && q != e &&
+          dist2(nx, ny, nz, px[j], py[j], pz[j], rp.lx, rp.ly, rp.lz, rp.hx, rp.hy, rp.hz) < 1.0)
         ov = true;
     }
     const unsigned long long bal = __ballot(ov);
     const unsigned long long mine =
         (LPC == 64) ? bal : ((bal >> (threadIdx.x & 32)) & 0xFFFFFFFFull);
     if (mine == 0ull) {
-      if (lane == 0) {
-        Lx[e] = nx;
-        Ly[e] = ny;
-        Lz[e] = nz;
+      if (lane == owner) {
+        px[je] = nx;
+        py[je] = ny;
+        pz[je] = nz;
       }
       ++n_acc;
     }
-    wave_lds_sync();
   }
-  // write the active cell's spheres back (slot copy and by-id arrays)
-  if (lane < nc) {
-    const int e = oc + lane;
-    const int s = Ss[lane];
-    const double px = Lx[e], py = Ly[e], pz = Lz[e];
-    sx[s] = px;
-    sy[s] = py;
-    sz[s] = pz;
-    const int g = rp.poff + Sp[lane];
-    x[g] = px;
-    y[g] = py;
-    z[g] = pz;
+  // write the active cell's spheres back (slot copy and by-id arrays),
+  // through to the device-coherent level so no dirty L2 lines are left for
+  // the end-of-kernel write-back
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    const int q = lane + j * LPC;
+    if (q >= oc && q < oc + nc) {
+      const int s = Lm[q];
+      store_through(&sx[s], (double)px[j]);
+      store_through(&sy[s], (double)py[j]);
+      store_through(&sz[s], (double)pz[j]);
+      const int g = rp.poff + Sp[q - oc];
+      store_through(&x[g], (double)px[j]);
+      store_through(&y[g], (double)py[j]);
+      store_through(&z[g], (double)pz[j]);
+    }
   }
-  // per-cell move counters: this work item is the cell's only writer
-  if (lane == 13 && (n_acc | n_left)) {
-    const int c = rp.coff + cid;
-    if (n_acc) cacc[c] += n_acc;
-    if (n_left) cleft[c] += n_left;
+  // move counters per active-cell index (the cells of a workgroup share a
+  // cache line); the adds never contend and their results are not waited for
+  if (lane == 13) {
+    if (n_acc) atomicAdd(&cacc[a], n_acc);
+    if (n_left) atomicAdd(&cleft[a], n_left);
   }
 }
 
@@ -293,9 +319,6 @@ PhaseFn phase_fn(int cap, int lpc, int lcap) {
   return phase_fn_l<16, 256>(lpc);
 }
 
-size_t phase_lds_bytes(int cap, int lcap, int cpb) {
-  return (size_t)cpb * (3 * (size_t)lcap * sizeof(double) + 3 * (size_t)cap * sizeof(int));
-}
 
 bool tuning_ok(const hsmc_gpu_ctx *ctx, const hsmc_gpu_tuning &t) {
   if (t.lanes_per_cell != 32 && t.lanes_per_cell != 64) return false;
@@ -368,6 +391,14 @@ int run_sweeps(hsmc_gpu_ctx *ctx, int n) {
 }
 
 }  // namespace
+
+namespace hsmc {
+// per cell: the slot map of up to lcap stencil spheres and the id and id
+// order of the cell's own occupants
+size_t phase_lds_bytes(int cap, int lcap, int cpb) {
+  return (size_t)cpb * ((size_t)lcap + 2 * (size_t)cap) * sizeof(int);
+}
+}  // namespace hsmc
 
 extern "C" {
 

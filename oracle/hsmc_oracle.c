@@ -708,10 +708,11 @@ void orc_cb_sweep_randoms(const orc_cb *c, uint64_t sweep, double shift[3], int 
   }
 }
 
-static inline int cb_axis_cell(double x, double shift, double L, double side, int g) {
+/* cell index on the shifted grid: (int)((x - shift, wrapped) * (1/side)) */
+static inline int cb_axis_cell(double x, double shift, double L, double inv_side, int g) {
   double t = x - shift;
   if (t < 0.0) t += L;
-  int ci = (int)(t / side);
+  int ci = (int)(t * inv_side);
   if (ci >= g) ci = g - 1;
   return ci;
 }
@@ -730,6 +731,7 @@ int orc_cb_sweeps(orc_cb *c, int n_sweeps) {
   const double lx = c->lx, ly = c->ly, lz = c->lz;
   const double hx = lx / 2.0, hy = ly / 2.0, hz = lz / 2.0;
   const double sx = lx / gx, sy = ly / gy, sz = lz / gz;
+  const double ix = 1.0 / sx, iy = 1.0 / sy, iz = 1.0 / sz;
   int *cnt = (int *)malloc(sizeof(int) * (size_t)(C + 1));
   int *start = (int *)malloc(sizeof(int) * (size_t)(C + 1));
   int *order = (int *)malloc(sizeof(int) * (size_t)N);
@@ -742,9 +744,9 @@ int orc_cb_sweeps(orc_cb *c, int n_sweeps) {
     /* bin: cells hold their occupants in ascending particle id */
     memset(cnt, 0, sizeof(int) * (size_t)(C + 1));
     for (int i = 0; i < N; i++) {
-      int cx = cb_axis_cell(c->x[i], shift[0], lx, sx, gx);
-      int cy = cb_axis_cell(c->y[i], shift[1], ly, sy, gy);
-      int cz = cb_axis_cell(c->z[i], shift[2], lz, sz, gz);
+      int cx = cb_axis_cell(c->x[i], shift[0], lx, ix, gx);
+      int cy = cb_axis_cell(c->y[i], shift[1], ly, iy, gy);
+      int cz = cb_axis_cell(c->z[i], shift[2], lz, iz, gz);
       cell[i] = (cx * gy + cy) * gz + cz;
       cnt[cell[i]]++;
     }
@@ -780,9 +782,9 @@ int orc_cb_sweeps(orc_cb *c, int n_sweeps) {
               if (ny > ly) ny -= ly; else if (ny < 0.0) ny += ly;
               if (nz > lz) nz -= lz; else if (nz < 0.0) nz += lz;
               c->attempts++;
-              if (cb_axis_cell(nx, shift[0], lx, sx, gx) != cx ||
-                  cb_axis_cell(ny, shift[1], ly, sy, gy) != cy ||
-                  cb_axis_cell(nz, shift[2], lz, sz, gz) != cz) {
+              if (cb_axis_cell(nx, shift[0], lx, ix, gx) != cx ||
+                  cb_axis_cell(ny, shift[1], ly, iy, gy) != cy ||
+                  cb_axis_cell(nz, shift[2], lz, iz, gz) != cz) {
                 c->left_cell++;
                 continue;
               }
