@@ -84,6 +84,9 @@ struct hsmc_gpu_ctx {
   // configuration version (bumped by anything that moves particles) and the
   // cached observable grid built for (replica, cutoff, version)
   unsigned long long version = 1;
+  // configuration version the sweep grid's cell list (the last sweep's count
+  // buffer and slots) describes; 0 when it describes none
+  unsigned long long cells_version = 0;
   int aux_replica = -1;
   double aux_cut = 0.0;
   unsigned long long aux_version = 0;
@@ -93,7 +96,7 @@ struct hsmc_gpu_ctx {
   unsigned long long ngrid_version[2] = {0, 0};
   double ngrid_cut[2] = {0.0, 0.0};
   // scratch buffers of the observables, grown on demand
-  static constexpr int kScratch = 40;
+  static constexpr int kScratch = 44;
   void *scratch[kScratch] = {};
   size_t scratch_bytes[kScratch] = {};
 };
@@ -108,6 +111,11 @@ void *scratch(hsmc_gpu_ctx *ctx, int slot, size_t bytes);
 int layout(hsmc_gpu_ctx *ctx);
 // LDS bytes of one sublattice-phase workgroup of cpb cells
 size_t phase_lds_bytes(int cap, int lcap, int cpb);
+// n checkerboard sweeps of every replica (enqueued on the context stream)
+int run_sweeps(hsmc_gpu_ctx *ctx, int n);
+// File the current by-id positions into the cell slots of the last sweep's
+// grid (the cell list the volume-move scan reads after positions changed).
+int rebin_last_sweep(hsmc_gpu_ctx *ctx);
 // Sweep-grid cells per axis for a box edge: largest even g with L/g > 1.
 int sweep_grid_cells(double L);
 // Read and clear the kernels' sticky error bits; translate to a status.

@@ -81,7 +81,71 @@ int fold_counters(hsmc_gpu_ctx *ctx) {
   return HSMC_OK;
 }
 
+// Capacity of the phase kernel's stencil list: spheres of unit diameter whose
+// centres lie in the 27-cell stencil fit in a box one diameter wider; their
+// count is bounded by the close-packing fraction pi/(3 sqrt 2) = 0.7405
+// (taken as 0.78 for margin) times that volume over the sphere volume, and
+// by N.
+long stencil_list_bound(const RepDev &d) {
+  const double span_x = std::min(3, d.gx) * d.sx + 1.0;
+  const double span_y = std::min(3, d.gy) * d.sy + 1.0;
+  const double span_z = std::min(3, d.gz) * d.sz + 1.0;
+  long b = (long)ceil(0.78 * span_x * span_y * span_z / (M_PI / 6.0));
+  return b > d.N ? d.N : b;
+}
+
+int lcap_for(long bound) { return bound <= 128 ? 128 : (bound <= 192 ? 192 : 256); }
+
+// Device parameters of one replica from its box, grid and offsets.
+void fill_repdev(const RepHost &h, RepDev &d) {
+  const hsmc_gpu_replica &p = h.p;
+  d.lx = p.lx; d.ly = p.ly; d.lz = p.lz;
+  d.hx = p.lx / 2.0; d.hy = p.ly / 2.0; d.hz = p.lz / 2.0;
+  d.sx = p.lx / h.gx; d.sy = p.ly / h.gy; d.sz = p.lz / h.gz;
+  d.ix = 1.0 / d.sx; d.iy = 1.0 / d.sy; d.iz = 1.0 / d.sz;
+  d.dr_max = p.dr_max;
+  d.N = p.n_part;
+  d.gx = h.gx; d.gy = h.gy; d.gz = h.gz;
+  d.poff = (int)h.poff; d.coff = (int)h.coff; d.aoff = (int)h.aoff;
+  d.k0 = (uint32_t)p.seed;
+  d.k1 = (uint32_t)(p.seed >> 32);
+}
+
+// A cube of side < 2/sqrt(3) holds at most 8 points 1 apart; wider cells get
+// 16 slots.
+bool needs_big_cells(const RepDev &d) { return d.sx >= 1.1547 || d.sy >= 1.1547 || d.sz >= 1.1547; }
+
+// Box or step-size change that keeps every replica's sweep grid, the slot
+// capacity and the stencil-list bound: only the parameter table changes.
+// The cell arrays, counters and a captured graph stay valid (the kernels
+// read the table at run time; the next sweep re-bins every sphere).
+bool layout_in_place(hsmc_gpu_ctx *ctx) {
+  if (!ctx->d_cnt || !ctx->d_reps) return false;
+  std::vector<RepDev> rd(ctx->R);
+  bool big = false;
+  long tmax = 0;
+  for (int r = 0; r < ctx->R; ++r) {
+    const RepHost &h = ctx->reps[r];
+    const hsmc_gpu_replica &p = h.p;
+    if (sweep_grid_cells(p.lx) != h.gx || sweep_grid_cells(p.ly) != h.gy ||
+        sweep_grid_cells(p.lz) != h.gz)
+      return false;
+    RepDev &d = rd[r];
+    fill_repdev(h, d);
+    big = big || needs_big_cells(d);
+    tmax = std::max(tmax, stencil_list_bound(d));
+  }
+  if ((big ? 16 : 8) != ctx->cap || lcap_for(tmax) != ctx->lcap) return false;
+  if (hipMemcpyAsync(ctx->d_reps, rd.data(), sizeof(RepDev) * ctx->R, hipMemcpyHostToDevice,
+                     ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess)  // rd is freed on return
+    return false;
+  ctx->version++;
+  return true;
+}
+
 int layout(hsmc_gpu_ctx *ctx) {
+  if (layout_in_place(ctx)) return HSMC_OK;
   {
     int st = fold_counters(ctx);  // the cell arrays may move
     if (st) return st;
@@ -106,39 +170,17 @@ int layout(hsmc_gpu_ctx *ctx) {
     coff += cells;
     aoff += cells / 8;
     RepDev &d = rd[r];
-    d.lx = p.lx; d.ly = p.ly; d.lz = p.lz;
-    d.hx = p.lx / 2.0; d.hy = p.ly / 2.0; d.hz = p.lz / 2.0;
-    d.sx = p.lx / h.gx; d.sy = p.ly / h.gy; d.sz = p.lz / h.gz;
-    d.ix = 1.0 / d.sx; d.iy = 1.0 / d.sy; d.iz = 1.0 / d.sz;
-    // a cube of side < 2/sqrt(3) holds at most 8 points 1 apart
-    if (d.sx >= 1.1547 || d.sy >= 1.1547 || d.sz >= 1.1547) big = true;
-    d.dr_max = p.dr_max;
-    d.N = p.n_part;
-    d.gx = h.gx; d.gy = h.gy; d.gz = h.gz;
-    d.poff = (int)h.poff; d.coff = (int)h.coff; d.aoff = (int)h.aoff;
-    d.k0 = (uint32_t)p.seed;
-    d.k1 = (uint32_t)(p.seed >> 32);
+    fill_repdev(h, d);
+    big = big || needs_big_cells(d);
     ao[r] = (int)h.aoff;
   }
   ao[ctx->R] = (int)aoff;
   if (coff > (1L << 30)) return HSMC_ERR_NOT_SUPPORTED;
-  // Capacity of the phase kernel's LDS list of neighbour spheres: spheres of
-  // unit diameter whose centres lie in the 27-cell stencil fit in a box one
-  // diameter wider; their count is bounded by the close-packing fraction
-  // pi/(3 sqrt 2) = 0.7405 (taken as 0.78 for margin) times that volume over
-  // the sphere volume, and by N.
+  // capacity of the phase kernel's stencil list
   {
     long tmax = 0;
-    for (int r = 0; r < ctx->R; ++r) {
-      const RepDev &d = rd[r];
-      const double span_x = std::min(3, d.gx) * d.sx + 1.0;
-      const double span_y = std::min(3, d.gy) * d.sy + 1.0;
-      const double span_z = std::min(3, d.gz) * d.sz + 1.0;
-      long b = (long)ceil(0.78 * span_x * span_y * span_z / (M_PI / 6.0));
-      if (b > d.N) b = d.N;
-      if (b > tmax) tmax = b;
-    }
-    ctx->lcap = tmax <= 128 ? 128 : (tmax <= 192 ? 192 : 256);
+    for (int r = 0; r < ctx->R; ++r) tmax = std::max(tmax, stencil_list_bound(rd[r]));
+    ctx->lcap = lcap_for(tmax);
   }
   if ((size_t)aoff > ctx->arep_alloc) {
     (void)hipFree(ctx->d_arep);

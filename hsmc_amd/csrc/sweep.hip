@@ -77,7 +77,7 @@ k_rebin(const RepDev *__restrict__ reps, const int *__restrict__ prep, int P, in
         double *__restrict__ sy, double *__restrict__ sz, int cap,
         const unsigned long long *__restrict__ sweep_base, int k, unsigned *__restrict__ err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const unsigned long long sweep = *sweep_base + (unsigned long long)k;
+  const unsigned long long sweep = *sweep_base + (unsigned long long)(long long)k;
   const int buf = (int)(sweep & 1ull);
   if (i < C) cnt[(size_t)(buf ^ 1) * C + i] = 0;  // next sweep's buffer
   if (i >= P) return;
@@ -368,6 +368,10 @@ int capture_graph(hsmc_gpu_ctx *ctx, int K) {
   return HSMC_OK;
 }
 
+
+}  // namespace
+
+namespace hsmc {
 int run_sweeps(hsmc_gpu_ctx *ctx, int n) {
   if (n <= 0) return HSMC_OK;
   ctx->version++;
@@ -387,12 +391,29 @@ int run_sweeps(hsmc_gpu_ctx *ctx, int n) {
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, ctx->stream, ctx->d_sweep, rem);
     HSMC_HIP(ctx, hipGetLastError());
   }
+  ctx->cells_version = ctx->version;  // the last sweep's cells hold every sphere
   return HSMC_OK;
 }
 
-}  // namespace
+int rebin_last_sweep(hsmc_gpu_ctx *ctx) {
+  // the last sweep done is *d_sweep - 1; its count buffer is refilled from
+  // zero and the rebin clears the other buffer as a sweep's rebin does
+  unsigned long long next = 0;
+  HSMC_HIP(ctx, hipMemcpyAsync(&next, ctx->d_sweep, sizeof(next), hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const int buf = (int)((next - 1ull) & 1ull);
+  HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cnt + (size_t)buf * ctx->Ctot, 0, sizeof(int) * ctx->Ctot,
+                               ctx->stream));
+  const long work = std::max(ctx->Ptot, ctx->Ctot);
+  hipLaunchKernelGGL(k_rebin, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, ctx->stream,
+                     ctx->d_reps, ctx->d_prep, (int)ctx->Ptot, (int)ctx->Ctot, ctx->d_x, ctx->d_y,
+                     ctx->d_z, ctx->d_cnt, ctx->d_spid, ctx->d_sx, ctx->d_sy, ctx->d_sz, ctx->cap,
+                     ctx->d_sweep, -1, ctx->d_err);
+  HSMC_HIP(ctx, hipGetLastError());
+  ctx->cells_version = ctx->version;
+  return HSMC_OK;
+}
 
-namespace hsmc {
 // per cell: the slot map of up to lcap stencil spheres and the id and id
 // order of the cell's own occupants
 size_t phase_lds_bytes(int cap, int lcap, int cpb) {
@@ -444,6 +465,7 @@ int hsmc_gpu_time_phase_kernel(hsmc_gpu_ctx *ctx, int n_sweeps, float *avg_ms, i
   }
   for (int i = 0; i < 16; ++i) (void)hipEventDestroy(ev[i]);
   ctx->version++;
+  ctx->cells_version = ctx->version;
   *avg_ms = (float)(total / count);
   if (launches) *launches = count;
   return check_device_errors(ctx);
@@ -555,6 +577,7 @@ int hsmc_gpu_autotune(hsmc_gpu_ctx *ctx, int sweeps_per_trial, hsmc_gpu_tuning *
   HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cleft, 0, sizeof(unsigned int) * ctx->cell_alloc, ctx->stream));
   HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cnt, 0, sizeof(int) * 2 * ctx->cell_alloc, ctx->stream));
   HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->version++;  // positions restored, cell lists cleared
   ctx->tot_acc = acc0;
   ctx->tot_left = left0;
   if (best) *best = winner;

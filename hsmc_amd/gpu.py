@@ -25,7 +25,8 @@ EXPORTS = [
     "hsmc_gpu_sync", "hsmc_gpu_upload", "hsmc_gpu_download", "hsmc_gpu_set_box",
     "hsmc_gpu_scale", "hsmc_gpu_grid", "hsmc_gpu_sweeps", "hsmc_gpu_time_sweeps",
     "hsmc_gpu_counters", "hsmc_gpu_reset_counters", "hsmc_gpu_set_sweep_counter",
-    "hsmc_gpu_set_tuning",
+    "hsmc_gpu_set_tuning", "hsmc_gpu_box", "hsmc_gpu_npt_sweeps",
+    "hsmc_gpu_any_overlap_scaled_batch",
     "hsmc_gpu_get_tuning", "hsmc_gpu_autotune", "hsmc_gpu_time_phase_kernel",
     "hsmc_gpu_cell_list", "hsmc_gpu_overlap_batch", "hsmc_gpu_overlap_scaled",
     "hsmc_gpu_any_overlap_scaled", "hsmc_gpu_pair_dist", "hsmc_gpu_rdf_counts",
@@ -93,6 +94,9 @@ def load() -> C.CDLL:
         "hsmc_gpu_counters": (i32, [vp, i32, P(u64)]),
         "hsmc_gpu_reset_counters": (i32, [vp]),
         "hsmc_gpu_set_sweep_counter": (i32, [vp, u64]),
+        "hsmc_gpu_box": (i32, [vp, i32, P(f64)]),
+        "hsmc_gpu_npt_sweeps": (i32, [vp, i32, P(f64), P(f64), P(u64)]),
+        "hsmc_gpu_any_overlap_scaled_batch": (i32, [vp, P(f64), P(C.c_uint8)]),
         "hsmc_gpu_set_tuning": (i32, [vp, P(Tuning)]),
         "hsmc_gpu_get_tuning": (i32, [vp, P(Tuning)]),
         "hsmc_gpu_autotune": (i32, [vp, i32, P(Tuning)]),
@@ -150,6 +154,7 @@ class Context:
         self.h = h
         self.L = L
         self.replicas = [dict(r) for r in replicas]
+        self.R = len(replicas)
 
     def close(self) -> None:
         if getattr(self, "h", None):
@@ -203,9 +208,26 @@ class Context:
         self._ck(self.L.hsmc_gpu_grid(self.h, r, _p(g, C.c_int32)), "grid")
         return tuple(int(v) for v in g)
 
+    def box(self, r: int = 0) -> tuple:
+        b = np.zeros(3, dtype=np.float64)
+        self._ck(self.L.hsmc_gpu_box(self.h, r, _p(b, C.c_double)), "box")
+        return tuple(float(v) for v in b)
+
     # --- sweeps ---
     def sweeps(self, n: int) -> None:
         self._ck(self.L.hsmc_gpu_sweeps(self.h, int(n)), "sweeps")
+
+    def npt_sweeps(self, n: int, press, dv_max) -> np.ndarray:
+        """n NpT sweeps of every replica; press and dv_max per replica (or one
+        value for all).  Returns per replica (proposed, accepted, rejected for
+        overlap, rejected by the rule) volume moves of these sweeps."""
+        R = self.R
+        p = np.ascontiguousarray(np.broadcast_to(np.asarray(press, dtype=np.float64), (R,)))
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dv_max, dtype=np.float64), (R,)))
+        out = np.zeros((R, 4), dtype=np.uint64)
+        self._ck(self.L.hsmc_gpu_npt_sweeps(self.h, int(n), _p(p, C.c_double), _p(d, C.c_double),
+                                            _p(out, C.c_uint64)), "npt_sweeps")
+        return out
 
     def time_sweeps(self, n: int) -> float:
         ms = C.c_float(0)
@@ -279,6 +301,15 @@ class Context:
         out = np.zeros(len(s), dtype=np.uint8)
         self._ck(self.L.hsmc_gpu_any_overlap_scaled(self.h, r, _p(s, C.c_double), len(s),
                                                     _p(out, C.c_uint8)), "any_overlap_scaled")
+        return out
+
+    def any_overlap_scaled_batch(self, sf) -> np.ndarray:
+        """Per replica r: 1 when no pair overlaps after scaling by sf[r]."""
+        s = np.ascontiguousarray(np.broadcast_to(np.asarray(sf, dtype=np.float64), (self.R,)))
+        out = np.zeros(self.R, dtype=np.uint8)
+        self._ck(self.L.hsmc_gpu_any_overlap_scaled_batch(self.h, _p(s, C.c_double),
+                                                          _p(out, C.c_uint8)),
+                 "any_overlap_scaled_batch")
         return out
 
     def pair_dist(self, i, j, sf: float = 1.0, r: int = 0) -> np.ndarray:

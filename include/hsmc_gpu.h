@@ -100,6 +100,8 @@ int hsmc_gpu_scale(hsmc_gpu_ctx *ctx, int replica, double sf,
 int hsmc_gpu_set_dr_max(hsmc_gpu_ctx *ctx, int replica, double dr_max);
 /* Sweep grid of the replica (cells per axis). */
 int hsmc_gpu_grid(const hsmc_gpu_ctx *ctx, int replica, int32_t out_g[3]);
+/* Box edges of the replica (sim_box_info_get's lx, ly, lz, sim_info.c:179). */
+int hsmc_gpu_box(const hsmc_gpu_ctx *ctx, int replica, double out[3]);
 
 /* ---- sweeps (nvt.c:201-209 sweep_nvt; moves.c:27-80 part_move) ---------- */
 
@@ -120,6 +122,17 @@ int hsmc_gpu_reset_counters(hsmc_gpu_ctx *ctx);
  * run continues the same streams (the restart of io_config.c:28-130 keeps
  * the MT19937 state for the same purpose).  Attempt counts are unchanged. */
 int hsmc_gpu_set_sweep_counter(hsmc_gpu_ctx *ctx, uint64_t sweep);
+/* NpT sweeps of every replica (sweep_npt, npt.c:177-196, batched): each sweep
+ * is one checkerboard sweep followed by the replica's Binomial(N, 1/(N+1))
+ * volume moves (vol_move, moves.c:83-153): ln V' = ln V + (r - 0.5) dv_max[r],
+ * sf = (V'/V)^(1/3); an overlap after scaling rejects, else r' <
+ * exp(-press[r] (V' - V) + (N+1) ln(V'/V)) accepts and scales positions and
+ * box by sf.  press[r] is beta P.  The volume draws come from a Philox stream
+ * per replica keyed by its seed, so a replica's run does not depend on the
+ * batch.  vol_counts (4 per replica, may be NULL) accumulates proposed,
+ * accepted, rejected for overlap, rejected by the acceptance rule. */
+int hsmc_gpu_npt_sweeps(hsmc_gpu_ctx *ctx, int n_sweeps, const double *press,
+                        const double *dv_max, uint64_t *vol_counts);
 int hsmc_gpu_set_tuning(hsmc_gpu_ctx *ctx, const hsmc_gpu_tuning *t);
 int hsmc_gpu_get_tuning(const hsmc_gpu_ctx *ctx, hsmc_gpu_tuning *t);
 /* Time the candidate launch shapes on this context's own replicas (their
@@ -156,6 +169,13 @@ int hsmc_gpu_overlap_scaled(hsmc_gpu_ctx *ctx, int replica, const int32_t *idx,
  * (compute_press.c:243-273), as one all-particle reduction. */
 int hsmc_gpu_any_overlap_scaled(hsmc_gpu_ctx *ctx, int replica, const double *sf,
                                 int n_sf, uint8_t *no_overlap);
+/* The same scan for every replica at once: no_overlap[r] = 1 when no pair of
+ * replica r overlaps after scaling by sf[r] (replicas with sf[r] <= 0 are
+ * not scanned and report 1).  One kernel serves the replicas whose sweep-grid
+ * cells are at least 1/sf[r] wide, reading the cell list the last sweep left;
+ * the others are scanned one by one as above.  The volume-move scan of
+ * batched NpT (vol_move, moves.c:104-112). */
+int hsmc_gpu_any_overlap_scaled_batch(hsmc_gpu_ctx *ctx, const double *sf, uint8_t *no_overlap);
 /* compute_dist (moves.c:400-431) for pairs (i[k], j[k]) with scale sf. */
 int hsmc_gpu_pair_dist(hsmc_gpu_ctx *ctx, int replica, const int32_t *i,
                        const int32_t *j, int m, double sf, double *out);

@@ -1,0 +1,122 @@
+"""Batched NpT (hsmc_gpu_npt_sweeps) and its volume-move overlap scan.
+
+* the batched scan (one kernel over the cell list the last sweep left) gives
+  the same decision as the single-replica scan, which the golden vectors pin
+  to the reference's check_overlap with sf (moves.c:104-112), on both of its
+  paths (sweep-grid cells at least 1/sf wide, and the fallback grid);
+* a replica's NpT run in a batch equals the same replica run alone, bit for
+  bit (its sweeps and volume draws use only its own seed);
+* the batched run samples the same density as the reference's serial NpT
+  chain (npt.c:177-196) at beta P = 3.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from hsmc_amd import gpu
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def make(specs):
+    """specs: (fcc cells per side, rho, seed, dr_max) per replica."""
+    reps, confs = [], []
+    for n, rho, seed, dr in specs:
+        conf, box = gpu.lattice(2, n, rho)
+        reps.append(dict(n=len(conf), box=box, dr_max=dr, seed=seed))
+        confs.append(conf)
+    ctx = gpu.Context(reps)
+    for r, c in enumerate(confs):
+        ctx.upload(c, r)
+    return ctx
+
+
+SCAN_SPECS = [(4, 0.80, 1, 0.1), (5, 0.85, 2, 0.1), (4, 0.88, 3, 0.1), (6, 0.84, 4, 0.1),
+              (3, 0.80, 5, 0.1), (5, 0.86, 6, 0.1)]
+
+
+def test_batched_scan_matches_single_replica_scan():
+    ctx = make(SCAN_SPECS)
+    ctx.sweeps(200)
+    sides = [min(b / g for b, g in zip(ctx.box(r), ctx.grid(r))) for r in range(ctx.R)]
+    outcomes, paths = set(), set()
+    for sf in (1.0, 0.99995, 0.9999, 0.9995, 0.999, 0.998, 0.99, 0.95, 0.9):
+        got = ctx.any_overlap_scaled_batch(sf)
+        for r in range(ctx.R):
+            assert got[r] == ctx.any_overlap_scaled([sf], r)[0], (sf, r)
+            outcomes.add(int(got[r]))
+            paths.add(sides[r] >= (1.0 / sf) * (1.0 + 1e-9))
+    assert outcomes == {0, 1} and paths == {True, False}
+    # a different factor per replica, and replicas left out (sf <= 0)
+    sfs = np.array([0.999, 0.0, 0.9995, 1.0, -1.0, 0.998])
+    got = ctx.any_overlap_scaled_batch(sfs)
+    for r in range(ctx.R):
+        want = 1 if sfs[r] <= 0 else ctx.any_overlap_scaled([sfs[r]], r)[0]
+        assert got[r] == want, r
+    # positions changed since the last sweep (a rescale): the cells are rebuilt
+    ctx.scale(1.001, tuple(v * 1.001 for v in ctx.box(2)), r=2)
+    for sf in (0.999, 0.9985):
+        got = ctx.any_overlap_scaled_batch(sf)
+        for r in range(ctx.R):
+            assert got[r] == ctx.any_overlap_scaled([sf], r)[0], (sf, r)
+    ctx.close()
+
+
+NPT_SPECS = [(4, 0.60, 11, 0.12), (5, 0.50, 12, 0.15), (3, 0.70, 13, 0.10)]
+NPT_PRESS = [3.0, 2.0, 4.0]
+NPT_DV = [0.01, 0.02, 0.008]
+
+
+def test_npt_batched_equals_alone():
+    ctx = make(NPT_SPECS)
+    counts = ctx.npt_sweeps(40, NPT_PRESS, NPT_DV)
+    assert counts[:, 1].sum() > 0 and counts[:, 2].sum() + counts[:, 3].sum() > 0
+    assert np.array_equal(counts[:, 0], counts[:, 1] + counts[:, 2] + counts[:, 3])
+    for r, spec in enumerate(NPT_SPECS):
+        alone = make([spec])
+        c1 = alone.npt_sweeps(40, NPT_PRESS[r], NPT_DV[r])
+        assert np.array_equal(c1[0], counts[r])
+        assert alone.box(0) == ctx.box(r)
+        assert np.array_equal(alone.download(0), ctx.download(r))
+        alone.close()
+    assert all(ctx.any_overlap_scaled_batch(1.0) == 1)
+    ctx.close()
+
+
+def test_npt_batched_density_matches_serial():
+    """12 replicas of N = 256 at beta P = 3 against 4 runs of the serial chain;
+    error bars from the run-to-run scatter of run means."""
+    R, eq, stat, every = 12, 2000, 6000, 5
+    ctx = make([(4, 0.63, 1 + r, 0.15) for r in range(R)])
+    ctx.npt_sweeps(eq, 3.0, 0.008)
+    rho = np.zeros((stat // every, R))
+    vol_moves = np.zeros((R, 4), dtype=np.uint64)
+    for k in range(stat // every):
+        vol_moves += ctx.npt_sweeps(every, 3.0, 0.008)
+        for r in range(R):
+            b = ctx.box(r)
+            rho[k, r] = 256 / (b[0] * b[1] * b[2])
+    assert all(ctx.any_overlap_scaled_batch(1.0) == 1)
+    # about one volume move per sweep: Binomial(256, 1/257) per sweep
+    per_sweep = vol_moves[:, 0].astype(float) / stat
+    assert abs(per_sweep.mean() - 256 / 257) < 0.05
+    gpu_means = rho.mean(axis=0)
+    cpu_means = []
+    for seed in range(101, 105):
+        o = O.System(2, 4, rho=0.63, neigh_dr=1.2, max_part=10, dr_max=0.15, seed=seed)
+        o.set_params(0.15, 0.008, 3.0)
+        o.sweep_npt(eq)
+        r_ = []
+        for _ in range(stat // every):
+            o.sweep_npt(every)
+            r_.append(o.box()["rho"])
+        cpu_means.append(np.mean(r_))
+    se_gpu = max(np.std(gpu_means, ddof=1), 0.0025) / math.sqrt(R)
+    se_cpu = max(np.std(cpu_means, ddof=1), 0.0025) / math.sqrt(len(cpu_means))
+    m_gpu, m_cpu = gpu_means.mean(), np.mean(cpu_means)
+    assert abs(m_gpu - m_cpu) < 4.0 * math.hypot(se_gpu, se_cpu), (m_gpu, m_cpu, se_gpu, se_cpu)
+    # Carnahan-Starling density at beta P = 3 is 0.6348 (N = 256 sits a little above)
+    assert abs(m_gpu - 0.6348) < 0.012
+    ctx.close()
