@@ -55,13 +55,16 @@ def traffic_from_profile():
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
-    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed sweeps (default: 20000 for n32000, 2000 for batched, 20 for npt)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed sweeps first (default: 1000, 200, 5 for the same workloads)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["n32000", "batched", "widom"], default="n32000",
+    ap.add_argument("--workload", choices=["n32000", "batched", "widom", "npt"], default="n32000",
                     help="n32000: the headline single system; batched: 256 replicas "
                          "N = 4 n^3, n in 3..15, rho in [0.30, 0.93]; widom: GPU seconds to "
-                         "sigma(beta mu_ex) = 0.01 at N = 108000, rho = 0.90, learned vs uniform")
+                         "sigma(beta mu_ex) = 0.01 at N = 108000, rho = 0.90, learned vs uniform; "
+                         "npt: 64 NpT replicas of N = 256000 at beta P in {5, 8, 11}")
     ap.add_argument("--sigma", type=float, default=0.01, help="widom: target sigma(beta mu_ex)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", type=str, default="",
@@ -72,6 +75,11 @@ def main() -> None:
                     help="only run one device-to-device copy of this many MiB (torch) and exit: "
                          "the known-size reference for rocprofv3 FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
+    defaults = {"n32000": (20000, 1000), "batched": (2000, 200), "widom": (0, 0), "npt": (20, 5)}
+    if args.steps is None:
+        args.steps = defaults[args.workload][0]
+    if args.warmup is None:
+        args.warmup = defaults[args.workload][1]
     if args.calibrate_copy_mib:
         import torch
         n = args.calibrate_copy_mib * (1 << 20) // 4
@@ -91,6 +99,8 @@ def main() -> None:
         return batched(args)
     if args.workload == "widom":
         return widom(args)
+    if args.workload == "npt":
+        return npt(args)
 
     from hsmc_amd import gpu
     conf, box = gpu.lattice(2, 20, 0.85)
@@ -192,6 +202,76 @@ def batched(args) -> None:
                                "in [0.30, 0.93], one Philox seed each", "total_spheres": total,
                    "tuning": tuning,
                    "acceptance": sum(a["accepted"] for a in acc) / max(1, sum(a["attempts"] for a in acc))},
+    }))
+
+
+def npt(args) -> None:
+    """64 NpT replicas of N = 256000 (fcc 4*40^3) at beta P = 5, 8, 11 (21, 21
+    and 22 replicas, each started at the Carnahan-Starling density of its
+    pressure), ln-volume step 0.001: one step is one NpT sweep of every
+    replica (a checkerboard sweep, then each replica's Binomial(N, 1/(N+1))
+    volume moves, each checked by an all-pairs scaled-overlap reduction)."""
+    import numpy as np
+
+    from hsmc_amd import analysis as A
+    from hsmc_amd import gpu
+
+    def cs_density(p):
+        lo, hi = 0.1, 0.99
+        for _ in range(100):
+            mid = 0.5 * (lo + hi)
+            if mid * A.carnahan_starling(mid)["Z"] < p:
+                lo = mid
+            else:
+                hi = mid
+        return 0.5 * (lo + hi)
+
+    levels = [5.0, 8.0, 11.0]
+    press = [levels[0]] * 21 + [levels[1]] * 21 + [levels[2]] * 22
+    R, n_cells, dv = len(press), 40, 0.001
+    reps, confs = [], []
+    for r, p in enumerate(press):
+        conf, box = gpu.lattice(2, n_cells, cs_density(p))
+        reps.append(dict(n=len(conf), box=box, dr_max=0.1, seed=1 + r))
+        confs.append(conf)
+    total = sum(r["n"] for r in reps)
+    ctx = gpu.Context(reps)
+    for r, c in enumerate(confs):
+        ctx.upload(c, r)
+    del confs
+    print(f"[bench] npt: {R} replicas, {total} spheres", file=sys.stderr, flush=True)
+    ctx.npt_sweeps(args.warmup, press, dv)
+    ctx.sync()
+    ctx.reset_counters()
+    t0 = time.perf_counter()
+    vol = ctx.npt_sweeps(args.steps, press, dv)
+    ctx.sync()
+    secs = time.perf_counter() - t0
+    acc = [ctx.counters(r) for r in range(R)]
+    attempts = sum(a["attempts"] for a in acc)
+    assert attempts == total * args.steps, (attempts, total * args.steps)
+    if not all(ctx.any_overlap_scaled_batch(1.0) == 1):
+        raise SystemExit("overlap found after the NpT sweeps")
+    dens = {}
+    for p in levels:
+        rs = [r for r in range(R) if press[r] == p]
+        vals = [reps[r]["n"] / float(np.prod(ctx.box(r))) for r in rs]
+        dens[str(p)] = {"mean_density": float(np.mean(vals)), "carnahan_starling": cs_density(p)}
+    print(json.dumps({
+        "metric": "MC move attempts/s (64 NpT replicas, N=256000, beta P in {5,8,11})",
+        "value": attempts / secs, "unit": "MC move attempts/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": secs * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (fcc 4*40^3 starts at the Carnahan-Starling density, Philox proposals)",
+        "config": {"workload": "64 NpT replicas (21 at beta P = 5, 21 at 8, 22 at 11), N = 256000 "
+                               "each, max ln-volume step 0.001, dr_max 0.1",
+                   "total_spheres": total, "tuning": ctx.get_tuning(),
+                   "acceptance": sum(a["accepted"] for a in acc) / attempts},
+        "volume_moves": {"proposed": int(vol[:, 0].sum()), "accepted": int(vol[:, 1].sum()),
+                         "rejected_overlap": int(vol[:, 2].sum()),
+                         "rejected_rule": int(vol[:, 3].sum()),
+                         "per_second": float(vol[:, 0].sum()) / secs},
+        "density_after_run": dens,
     }))
 
 
