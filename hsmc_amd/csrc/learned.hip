@@ -6,9 +6,10 @@
 // centres (FP32) -- and a score f_v from a 12->64->64->1 ReLU network whose
 // weights are BF16 and whose products run on the MFMA units with FP32
 // accumulation.  Insertions are drawn from q_v = mix * softmax(f)_v +
-// (1 - mix)/V, allocated to voxels by stratified rounding (voxel v receives
-// floor(n q_v) or one more, with mean exactly n q_v) and placed uniformly in
-// their voxel; the overlap test is the exact FP64 one.  The estimate
+// (1 - mix)/V by systematic sampling (insertion i falls in the voxel whose
+// cumulative-mass interval holds (i + u0)/n: voxel v receives floor(n q_v)
+// or one more, with mean exactly n q_v) and placed uniformly in their voxel;
+// the overlap test is the exact FP64 one.  The estimate
 // P = (1/(V n)) sum_hits 1/q_v of the insertion probability is unbiased for
 // any q with q_v > 0, so plain uniform insertion stays the reference.
 //
@@ -452,29 +453,69 @@ k_voxel_scores(NGrid g, Vox vx, float far2, const MlpDev *__restrict__ net,
   }
 }
 
-// Stratified allocation: voxel v gets floor(n q_v) insertions plus one with
-// probability frac(n q_v), so E[n_v] = n q_v exactly.
-__global__ void __launch_bounds__(256)
-k_allocate(const float *__restrict__ f, long V, double fmax, double sumexp, double mix, double n,
-           uint32_t k0, uint32_t k1, unsigned long long sample, int *__restrict__ cnt) {
-  const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= V) return;
-  const double q = mix * exp((double)f[v] - fmax) / sumexp + (1.0 - mix) / (double)V;
-  const double nq = n * q;
-  const double fl = floor(nq);
-  uint32_t u[4];
-  philox4x32_10((uint32_t)v, (uint32_t)(v >> 32), (uint32_t)sample,
-                0x4C570000u | (uint32_t)((sample >> 32) & 0xFFFFu), k0, k1, u);
-  const double r = ((double)u[0] + 0.5) * 0x1p-32;
-  cnt[v] = (int)fl + (r < nq - fl ? 1 : 0);
+// Proposal mass of voxel v: q_v = mix softmax(f)_v + (1 - mix) / V.
+__device__ __forceinline__ double proposal_mass(const float *__restrict__ f, long v, long V,
+                                                double fmax, double sumexp, double mix) {
+  return mix * exp((double)f[v] - fmax) / sumexp + (1.0 - mix) / (double)V;
 }
 
-__global__ void k_fill(const int *__restrict__ cnt, const int *__restrict__ off, long V,
-                       int *__restrict__ list) {
-  const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= V) return;
-  const int c = cnt[v], o = off[v];
-  for (int j = 0; j < c; ++j) list[o + j] = (int)v;
+// Cumulative proposal mass, in three passes: inclusive sums within blocks of
+// 4096 voxels (+ block totals), an exclusive scan of the block totals (one
+// workgroup; its last entry is the total), then the block offsets added.
+__global__ void __launch_bounds__(1024)
+k_cdf_local(const float *__restrict__ f, long V, double fmax, double sumexp, double mix,
+            double *__restrict__ cdf, double *__restrict__ bsum) {
+  __shared__ double s[1024];
+  const long base = (long)blockIdx.x * 4096;
+  double v[4], tot = 0.0;
+  for (int j = 0; j < 4; ++j) {
+    const long i = base + threadIdx.x * 4 + j;
+    v[j] = i < V ? proposal_mass(f, i, V, fmax, sumexp, mix) : 0.0;
+    tot += v[j];
+  }
+  s[threadIdx.x] = tot;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const double t = threadIdx.x >= d ? s[threadIdx.x - d] : 0.0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  double run = s[threadIdx.x] - tot;
+  for (int j = 0; j < 4; ++j) {
+    const long i = base + threadIdx.x * 4 + j;
+    run += v[j];
+    if (i < V) cdf[i] = run;
+  }
+  if (threadIdx.x == 1023) bsum[blockIdx.x] = s[1023];
+}
+
+__global__ void __launch_bounds__(1024) k_cdf_blocks(double *__restrict__ b, int nb) {
+  __shared__ double part[1024];
+  const int t = threadIdx.x, per = (nb + 1023) / 1024, lo = t * per, hi = min(nb, lo + per);
+  double sum = 0.0;
+  for (int i = lo; i < hi; ++i) sum += b[i];
+  part[t] = sum;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const double v = t >= d ? part[t - d] : 0.0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  double run = part[t] - sum;
+  for (int i = lo; i < hi; ++i) {
+    const double bi = b[i];
+    b[i] = run;
+    run += bi;
+  }
+  __syncthreads();
+  if (t == 1023) b[nb] = part[1023];
+}
+
+__global__ void k_cdf_add(double *__restrict__ cdf, long V, const double *__restrict__ boff) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < V) cdf[i] += boff[i / 4096];
 }
 
 __device__ __forceinline__ bool widom_free(const NGrid &g1, double qx, double qy, double qz) {
@@ -499,22 +540,34 @@ __device__ __forceinline__ bool widom_free(const NGrid &g1, double qx, double qy
   return true;
 }
 
-// Insertion i goes to voxel list[i], uniformly inside it; a hit adds its
-// importance weight 1/(V n q_v).
+// Systematic sampling of n insertions from the proposal: insertion i takes
+// the voxel whose cumulative mass interval holds (i + u0) / n * total (so a
+// voxel gets floor or ceil of n q_v insertions, n q_v on average), a point
+// uniform inside it, and on a hit the importance weight 1/(V n q_v).
 __global__ void __launch_bounds__(256)
-k_insert(NGrid g1, Vox vx, const int *__restrict__ list, long total, const float *__restrict__ f,
-         double fmax, double sumexp, double mix, double n, uint32_t k0, uint32_t k1,
-         unsigned long long sample, double *__restrict__ acc, unsigned long long *__restrict__ hits) {
+k_insert(NGrid g1, Vox vx, const double *__restrict__ cdf, double total, long n, double u0,
+         const float *__restrict__ f, double fmax, double sumexp, double mix, uint32_t k0,
+         uint32_t k1, unsigned long long sample, double *__restrict__ acc,
+         unsigned long long *__restrict__ hits) {
   __shared__ double rs[4];
   __shared__ unsigned long long rh[4];
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double w = 0.0;
   unsigned long long h = 0;
-  if (i < total) {
-    const long v = list[i];
+  if (i < n) {
+    const double t = ((double)i + u0) / (double)n * total;
+    long lo = 0, hi = vx.V - 1;  // first voxel whose cumulative mass exceeds t
+    while (lo < hi) {
+      const long mid = (lo + hi) >> 1;
+      if (cdf[mid] > t)
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    const long v = lo;
     const int iz = (int)(v % vx.mz);
-    const long t = v / vx.mz;
-    const int iy = (int)(t % vx.my), ix = (int)(t / vx.my);
+    const long tv = v / vx.mz;
+    const int iy = (int)(tv % vx.my), ix = (int)(tv / vx.my);
     uint32_t u[4];
     philox4x32_10((uint32_t)i, (uint32_t)(i >> 32), (uint32_t)sample,
                   0x4C490000u | (uint32_t)((sample >> 32) & 0xFFFFu), k0, k1, u);
@@ -522,8 +575,7 @@ k_insert(NGrid g1, Vox vx, const int *__restrict__ list, long total, const float
     const double qy = (iy + ((double)u[1] + 0.5) * 0x1p-32) * vx.vy;
     const double qz = (iz + ((double)u[2] + 0.5) * 0x1p-32) * vx.vz;
     if (widom_free(g1, qx, qy, qz)) {
-      const double q = mix * exp((double)f[v] - fmax) / sumexp + (1.0 - mix) / (double)vx.V;
-      w = 1.0 / ((double)vx.V * n * q);
+      w = 1.0 / ((double)vx.V * (double)n * (proposal_mass(f, v, vx.V, fmax, sumexp, mix) / total));
       h = 1;
     }
   }
@@ -544,39 +596,6 @@ k_insert(NGrid g1, Vox vx, const int *__restrict__ list, long total, const float
       atomicAdd(hits, sh);
     }
   }
-}
-
-// exclusive prefix sum of cnt[0..n) (int) into off[0..n], multi-block
-__global__ void __launch_bounds__(1024) k_scan_local(const int *__restrict__ in, long n,
-                                                     int *__restrict__ out, int *__restrict__ bsum) {
-  __shared__ int s[1024];
-  const long base = (long)blockIdx.x * 4096;
-  int v[4], tot = 0;
-  for (int j = 0; j < 4; ++j) {
-    const long i = base + threadIdx.x * 4 + j;
-    v[j] = i < n ? in[i] : 0;
-    tot += v[j];
-  }
-  s[threadIdx.x] = tot;
-  __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
-    const int t = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
-    __syncthreads();
-    s[threadIdx.x] += t;
-    __syncthreads();
-  }
-  int run = s[threadIdx.x] - tot;
-  for (int j = 0; j < 4; ++j) {
-    const long i = base + threadIdx.x * 4 + j;
-    if (i < n) out[i] = run;
-    run += v[j];
-  }
-  if (threadIdx.x == 1023) bsum[blockIdx.x] = s[1023];
-}
-
-__global__ void k_scan_add(int *__restrict__ out, long n, const int *__restrict__ boff) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] += boff[i / 4096];
 }
 
 // single-block exclusive scan of up to a few thousand entries (+ total)
@@ -620,6 +639,8 @@ struct hsmc_gpu_mlp {
   unsigned long long sc_version = 0;
   long sc_step = -1;
   double sc_voxel = 0.0, sc_fmax = 0.0, sc_sumexp = 0.0;
+  double sc_total = 0.0;  // total of the cached cumulative proposal mass (mix)
+  double sc_mix = -1.0;
 };
 
 namespace {
@@ -660,8 +681,8 @@ int grid_cells_for2(double L, double cut) {
 // ctx.hip 21..22, sweep.hip 23): descriptor grid 24..26, overlap grid 27..29,
 // rows / scores 30, targets / block partials 31, counts 32, offsets 33,
 // sums 34, insertion list 35, scan partials 36..37.
-enum { L_GRID_A = 24, L_GRID_B = 27, L_X = 30, L_T = 31, L_CNT = 32, L_OFF = 33, L_ACC = 34,
-       L_LIST = 35, L_BSUM = 36, L_BOFF = 37, L_F = 38, L_FRED = 39 };
+enum { L_GRID_A = 24, L_GRID_B = 27, L_X = 30, L_T = 31, L_CDF = 32, L_ACC = 34,
+       L_BSUM = 36, L_F = 38, L_FRED = 39 };
 
 int build_ngrid(hsmc_gpu_ctx *ctx, int replica, double cut, int slot0, NGrid &g) {
   const RepHost &h = ctx->reps[replica];
@@ -719,19 +740,6 @@ float far2_of(const NGrid &g) {
   return (float)(s * s);
 }
 
-int exclusive_scan(hsmc_gpu_ctx *ctx, const int *cnt, long V, int *off, int *total_out) {
-  const long nb = (V + 4095) / 4096;
-  int *bsum = (int *)scratch(ctx, L_BSUM, sizeof(int) * (nb + 1));
-  int *boff = (int *)scratch(ctx, L_BOFF, sizeof(int) * (nb + 1));
-  if (!bsum || !boff) return HSMC_ERR_NO_MEMORY;
-  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(1024), 0, ctx->stream, cnt, V, off, bsum);
-  hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, ctx->stream, bsum, (int)nb, boff);
-  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, ctx->stream, off, V, boff);
-  HSMC_HIP(ctx, hipGetLastError());
-  HSMC_HIP(ctx, hipMemcpyAsync(total_out, boff + nb, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return HSMC_OK;
-}
 
 }  // namespace
 
@@ -916,14 +924,16 @@ int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mi
   const Vox vx = make_vox(ctx->reps[replica], voxel);
   if (vx.V > (1L << 31) - 1) return HSMC_ERR_NOT_SUPPORTED;
   const long nblk = (vx.V + 255) / 256;
+  const long ncdf = (vx.V + 4095) / 4096;
   float *f = (float *)scratch(ctx, L_F, sizeof(float) * (size_t)vx.V);
   float *bred = (float *)scratch(ctx, L_FRED, sizeof(float) * 2 * (size_t)nblk);
-  int *cnt = (int *)scratch(ctx, L_CNT, sizeof(int) * (size_t)vx.V);
-  int *off = (int *)scratch(ctx, L_OFF, sizeof(int) * (size_t)(vx.V + 1));
+  double *cdf = (double *)scratch(ctx, L_CDF, sizeof(double) * (size_t)vx.V);
+  double *bsum = (double *)scratch(ctx, L_BSUM, sizeof(double) * (size_t)(ncdf + 1));
   double *acc = (double *)scratch(ctx, L_ACC, sizeof(double) * 2);
-  if (!f || !bred || !cnt || !off || !acc) return HSMC_ERR_NO_MEMORY;
-  if (!(m->sc_replica == replica && m->sc_version == ctx->version && m->sc_step == m->step &&
-        m->sc_voxel == voxel)) {
+  if (!f || !bred || !cdf || !bsum || !acc) return HSMC_ERR_NO_MEMORY;
+  const bool scores_ok = m->sc_replica == replica && m->sc_version == ctx->version &&
+                         m->sc_step == m->step && m->sc_voxel == voxel;
+  if (!scores_ok) {
     hipLaunchKernelGGL(k_voxel_scores, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, g, vx,
                        far2_of(g), m->net, f, bred, bred + nblk);
     HSMC_HIP(ctx, hipGetLastError());
@@ -940,30 +950,38 @@ int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mi
     m->sc_voxel = voxel;
     m->sc_fmax = fm;
     m->sc_sumexp = se;
+    m->sc_mix = -1.0;  // the cumulative mass is rebuilt below
   }
   const double fmax = m->sc_fmax, sumexp = m->sc_sumexp;
-  hipLaunchKernelGGL(k_allocate, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, f, vx.V, fmax,
-                     sumexp, mix, (double)n, (uint32_t)seed, (uint32_t)(seed >> 32),
-                     (unsigned long long)sample, cnt);
-  HSMC_HIP(ctx, hipGetLastError());
-  int total = 0;
-  st = exclusive_scan(ctx, cnt, vx.V, off, &total);
-  if (st) return st;
-  int *list = (int *)scratch(ctx, L_LIST, sizeof(int) * (size_t)(total + 1));
-  if (!list) return HSMC_ERR_NO_MEMORY;
-  hipLaunchKernelGGL(k_fill, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, cnt, off, vx.V, list);
+  if (m->sc_mix != mix) {
+    hipLaunchKernelGGL(k_cdf_local, dim3((unsigned)ncdf), dim3(1024), 0, ctx->stream, f, vx.V, fmax,
+                       sumexp, mix, cdf, bsum);
+    hipLaunchKernelGGL(k_cdf_blocks, dim3(1), dim3(1024), 0, ctx->stream, bsum, (int)ncdf);
+    hipLaunchKernelGGL(k_cdf_add, dim3((unsigned)((vx.V + 255) / 256)), dim3(256), 0, ctx->stream,
+                       cdf, vx.V, bsum);
+    HSMC_HIP(ctx, hipGetLastError());
+    double total = 0.0;
+    HSMC_HIP(ctx, hipMemcpyAsync(&total, bsum + ncdf, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    m->sc_total = total;
+    m->sc_mix = mix;
+  }
+  // one uniform offset per call for the systematic sample
+  uint32_t u[4];
+  philox4x32_10(0u, 0u, (uint32_t)sample, 0x4C530000u | (uint32_t)((sample >> 32) & 0xFFFFu),
+                (uint32_t)seed, (uint32_t)(seed >> 32), u);
+  const double u0 = (double)u[0] * 0x1p-32;
   HSMC_HIP(ctx, hipMemsetAsync(acc, 0, sizeof(double) * 2, ctx->stream));
-  if (total > 0)
-    hipLaunchKernelGGL(k_insert, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream,
-                       g1, vx, list, (long)total, f, fmax, sumexp, mix, (double)n, (uint32_t)seed,
-                       (uint32_t)(seed >> 32), (unsigned long long)sample, acc,
-                       reinterpret_cast<unsigned long long *>(acc + 1));
+  hipLaunchKernelGGL(k_insert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, g1, vx,
+                     cdf, m->sc_total, (long)n, u0, f, fmax, sumexp, mix, (uint32_t)seed,
+                     (uint32_t)(seed >> 32), (unsigned long long)sample, acc,
+                     reinterpret_cast<unsigned long long *>(acc + 1));
   HSMC_HIP(ctx, hipGetLastError());
   double ha[2] = {0, 0};
   HSMC_HIP(ctx, hipMemcpyAsync(ha, acc, sizeof(double) * 2, hipMemcpyDeviceToHost, ctx->stream));
   HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   *estimate = ha[0];
-  if (n_done) *n_done = total;
+  if (n_done) *n_done = n;
   if (hits) *hits = (int64_t)__builtin_bit_cast(unsigned long long, ha[1]);
   return HSMC_OK;
 }

@@ -236,10 +236,11 @@ int hsmc_gpu_mlp_train(hsmc_gpu_mlp *m, int replica, double voxel, uint64_t seed
 /* Scores of all voxels (cap >= *n_voxels; scores may be NULL to query V). */
 int hsmc_gpu_voxel_scores(hsmc_gpu_mlp *m, int replica, double voxel, float *scores,
                           int64_t cap, int64_t *n_voxels);
-/* Importance-sampled Widom: about n insertions from q = mix * softmax(score)
- * + (1 - mix) / V, stratified over voxels; *estimate = (1/(V n)) sum over
+/* Importance-sampled Widom: n insertions from q = mix * softmax(score)
+ * + (1 - mix) / V, by systematic sampling over voxels (each voxel gets floor
+ * or ceil of n q_v, n q_v on average); *estimate = (1/(V n)) sum over
  * insertable points of 1/q_v, an unbiased estimate of the insertion
- * probability; *n_done insertions made, *hits insertable. */
+ * probability; *n_done insertions made (n), *hits insertable. */
 int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mix,
                            uint64_t seed, uint64_t sample, int64_t n, double *estimate,
                            int64_t *n_done, int64_t *hits);

@@ -105,3 +105,18 @@ def test_trained_network_beats_uniform():
     gain = (su / sl) ** 2   # variance ratio at equal insertion counts
     print(f"rho=0.8 P_ins uniform {pu:.4e}+-{su:.1e} learned {pl:.4e}+-{sl:.1e} gain {gain:.1f}")
     assert gain > 2.0
+
+
+def test_systematic_sampler_weights():
+    """mix = 0 makes q uniform (q_v = 1/V): every insertion's weight is then
+    exactly 1/n, so the estimate equals hits / n, and all n insertions are
+    made whatever n is."""
+    ctx, L = _ctx_with(6, 0.5, 20)
+    net = gpu.Mlp(ctx, seed=3)
+    for n in (1, 777, 100000):
+        r = net.widom(0.25, 0.0, 5, 1, n)
+        assert r["n"] == n
+        assert abs(r["estimate"] - r["hits"] / n) <= 1e-12 * max(1.0, r["hits"] / n)
+    # with a proposal that is not uniform the count is still exactly n
+    r = net.widom(0.25, 0.9, 5, 2, 54321)
+    assert r["n"] == 54321 and r["hits"] > 0
