@@ -281,20 +281,21 @@ def _timed(fn):
     return out, time.perf_counter() - t0
 
 
-def widom(args) -> None:
-    """beta mu_ex at rho = 0.90, N = 108000 (fcc 4*30^3, melted): GPU seconds of
-    insertion work until the Monte Carlo error sigma(beta mu_ex) reaches the
-    target, plain uniform insertions against the learned proposal, on the same
-    K configurations (one context each).  Both estimates must agree within 2
-    sigma.  The network is trained once on 10^6 uniform insertions of the
-    first configuration (its time is reported apart)."""
+def _widom_density(args, rho, n_uniform, n_learned):
+    """GPU seconds of insertion work until sigma(beta mu_ex) reaches the
+    target at density rho, N = 108000 (fcc 4*30^3, melted by 10^4 sweeps),
+    plain uniform insertions against the learned proposal on the same K = 4
+    configurations (10^3 sweeps apart, one context each).  The network is
+    trained once on 10^6 uniform insertions of the first configuration (its
+    time is reported apart; the learned seconds include scoring every
+    configuration's voxels)."""
     import math
 
     import numpy as np
 
     from hsmc_amd import analysis as A
     from hsmc_amd import gpu
-    rho, n_cells, K = 0.90, 30, 4
+    n_cells, K = 30, 4
     voxel, mix = 0.25, 0.9
     conf, box = gpu.lattice(2, n_cells, rho)
     N = len(conf)
@@ -305,6 +306,7 @@ def widom(args) -> None:
     for k in range(K):
         base.sweeps(1000)
         confs.append(base.download())
+    base.close()
     ctxs = []
     for k in range(K):
         c = gpu.Context([dict(n=N, box=box, dr_max=0.1, seed=100 + k)])
@@ -312,7 +314,6 @@ def widom(args) -> None:
         ctxs.append(c)
     print(f"[bench] widom: {K} configurations of N={N} at rho={rho}", file=sys.stderr, flush=True)
 
-    # train on 10^6 uniform insertions of configuration 0
     net0 = gpu.Mlp(ctxs[0], seed=5)
     loss, t_train = _timed(lambda: net0.train(voxel, 777, 0, 1_000_000, epochs=10, batch=4096,
                                                lr=2e-3, beta=20.0))
@@ -322,7 +323,9 @@ def widom(args) -> None:
         m.set(params)
     print(f"[bench] trained: loss {loss:.4f} in {t_train:.3f} s", file=sys.stderr, flush=True)
 
-    def run(method, n_per, min_rounds=8, max_rounds=4000):
+    # at least 16 rounds per configuration, so the variance estimates behind
+    # sigma (and the agreement test) are not themselves too noisy
+    def run(method, n_per, min_rounds=16, max_rounds=4000):
         est = [[] for _ in range(K)]
         secs = 0.0
         n_total = 0
@@ -353,25 +356,48 @@ def widom(args) -> None:
                             insertions=n_total, rounds=rnd,
                             insertions_per_s=n_total / secs)
 
-    uni = run("uniform", 100_000_000)
-    print(f"[bench] uniform: {uni}", file=sys.stderr, flush=True)
-    lrn = run("learned", 1_000_000)
-    print(f"[bench] learned: {lrn}", file=sys.stderr, flush=True)
-    z = abs(uni["beta_mu"] - lrn["beta_mu"]) / math.hypot(uni["sigma"], lrn["sigma"])
-    cs = A.carnahan_starling(rho)
+    uni = run("uniform", n_uniform)
+    print(f"[bench] rho {rho} uniform: {uni}", file=sys.stderr, flush=True)
+    lrn = run("learned", n_learned)
+    print(f"[bench] rho {rho} learned: {lrn}", file=sys.stderr, flush=True)
+    for c in ctxs:
+        c.close()
+    return {
+        "rho": rho,
+        "ratio": uni["seconds"] / lrn["seconds"],
+        "ratio_at_target": uni["seconds_at_target"] / lrn["seconds_at_target"],
+        "agreement_sigmas": abs(uni["beta_mu"] - lrn["beta_mu"]) / math.hypot(uni["sigma"], lrn["sigma"]),
+        "carnahan_starling_beta_mu_ex": A.carnahan_starling(rho)["mu_ex"],
+        "uniform": uni, "learned": lrn, "learned_training_seconds": t_train,
+    }
+
+
+def widom(args) -> None:
+    """Learned against uniform Widom insertion at rho = 0.70, 0.80, 0.90
+    (N = 108000, voxel 0.25 sigma, mix 0.9); the headline value is rho = 0.90.
+    Insertions per call are sized to each density's insertion probability."""
+    sizes = {0.70: (1_000_000, 100_000), 0.80: (10_000_000, 250_000), 0.90: (50_000_000, 500_000)}
+    res = {str(rho): _widom_density(args, rho, *sizes[rho]) for rho in (0.70, 0.80, 0.90)}
+    top = res["0.9"]
     print(json.dumps({
         "metric": "GPU seconds to sigma(beta mu_ex)=0.01 at rho=0.90, learned vs uniform Widom",
-        "value": uni["seconds"] / lrn["seconds"], "unit": "x less GPU time (uniform / learned)",
+        "value": top["ratio"], "unit": "x less GPU time (uniform / learned)",
         "n_gpus": 1, "higher_is_better": True, "vs_baseline": None, "dtype": "f64 (overlap), bf16 (network)",
-        "data": "synthetic (fcc 4*30^3 at rho=0.90 melted by 10^4 sweeps; 4 configurations 10^3 sweeps apart)",
+        "data": "synthetic (fcc 4*30^3 melted by 10^4 sweeps; 4 configurations 10^3 sweeps apart per density)",
         "config": {"workload": "N=108000, rho=0.90, voxel 0.25 sigma (198^3), mix 0.9, "
                                "network 12-64-64-1 trained on 1e6 uniform insertions",
                    "target_sigma": args.sigma},
-        "ratio_at_target": uni["seconds_at_target"] / lrn["seconds_at_target"],
-        "uniform": uni, "learned": lrn, "learned_training_seconds": t_train,
-        "agreement_sigmas": z, "carnahan_starling_beta_mu_ex": cs["mu_ex"],
+        "ratio_at_target": top["ratio_at_target"],
+        "uniform": top["uniform"], "learned": top["learned"],
+        "learned_training_seconds": top["learned_training_seconds"],
+        "agreement_sigmas": top["agreement_sigmas"],
+        "carnahan_starling_beta_mu_ex": top["carnahan_starling_beta_mu_ex"],
+        "densities": {k: {kk: v[kk] for kk in ("ratio", "ratio_at_target", "agreement_sigmas",
+                                                "carnahan_starling_beta_mu_ex")}
+                      | {"beta_mu_uniform": v["uniform"]["beta_mu"], "beta_mu_learned": v["learned"]["beta_mu"],
+                         "seconds_uniform": v["uniform"]["seconds"], "seconds_learned": v["learned"]["seconds"]}
+                      for k, v in res.items()},
     }))
-
 
 if __name__ == "__main__":
     main()

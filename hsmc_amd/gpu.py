@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -155,9 +156,12 @@ class Context:
         self.L = L
         self.replicas = [dict(r) for r in replicas]
         self.R = len(replicas)
+        self._nets = weakref.WeakSet()   # networks to destroy before the context
 
     def close(self) -> None:
         if getattr(self, "h", None):
+            for m in list(getattr(self, "_nets", ())):
+                m.close()
             self.L.hsmc_gpu_destroy(self.h)
             self.h = None
 
@@ -359,6 +363,7 @@ class Mlp:
         h = C.c_void_p()
         ctx._ck(self.L.hsmc_gpu_mlp_create(ctx.h, C.byref(h), int(seed)), "mlp_create")
         self.h = h
+        ctx._nets.add(self)
 
     def close(self) -> None:
         if getattr(self, "h", None):

@@ -214,7 +214,8 @@ int hsmc_gpu_widom_uniform(hsmc_gpu_ctx *ctx, int replica, uint64_t seed,
 
 typedef struct hsmc_gpu_mlp hsmc_gpu_mlp;
 
-/* Network on ctx's device, He-uniform weights from Philox(seed). */
+/* Network on ctx's device, He-uniform weights from Philox(seed).  A network
+ * uses its context's stream and buffers: destroy it before the context. */
 int hsmc_gpu_mlp_create(hsmc_gpu_ctx *ctx, hsmc_gpu_mlp **out, uint64_t seed);
 int hsmc_gpu_mlp_destroy(hsmc_gpu_mlp *m);
 int hsmc_gpu_mlp_set(hsmc_gpu_mlp *m, const float *params);
