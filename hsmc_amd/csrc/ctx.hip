@@ -112,8 +112,11 @@ void fill_repdev(const RepHost &h, RepDev &d) {
 }
 
 // A cube of side < 2/sqrt(3) holds at most 8 points 1 apart; wider cells get
-// 16 slots.
+// 16 slots, which bound the occupancy of cubes narrower than 1.5 (an fcc
+// cell of side sqrt(2) holds 14 points 1 apart).  Wider cells only occur on
+// 2-cell axes of boxes under 4 diameters and are refused.
 bool needs_big_cells(const RepDev &d) { return d.sx >= 1.1547 || d.sy >= 1.1547 || d.sz >= 1.1547; }
+bool cells_too_wide(const RepDev &d) { return d.sx >= 1.5 || d.sy >= 1.5 || d.sz >= 1.5; }
 
 // Box or step-size change that keeps every replica's sweep grid, the slot
 // capacity and the stencil-list bound: only the parameter table changes.
@@ -132,6 +135,7 @@ bool layout_in_place(hsmc_gpu_ctx *ctx) {
       return false;
     RepDev &d = rd[r];
     fill_repdev(h, d);
+    if (cells_too_wide(d)) return false;  // refused by the full layout
     big = big || needs_big_cells(d);
     tmax = std::max(tmax, stencil_list_bound(d));
   }
@@ -171,6 +175,10 @@ int layout(hsmc_gpu_ctx *ctx) {
     aoff += cells / 8;
     RepDev &d = rd[r];
     fill_repdev(h, d);
+    if (cells_too_wide(d)) {
+      ctx->last_error = "sweep-grid cells of 1.5 diameters or more (a box edge in [3, 4)) are not supported";
+      return HSMC_ERR_NOT_SUPPORTED;
+    }
     big = big || needs_big_cells(d);
     ao[r] = (int)h.aoff;
   }
