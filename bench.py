@@ -62,6 +62,37 @@ def cpu_baseline(target_s: float = 12.0) -> dict:
                                     f"about 8 s of serial sweeps each"}}
 
 
+def observables(ctx, no_cpu: bool) -> dict:
+    """Per-sample cost of the observables on the same N = 32000 system, each
+    sample right after a sweep (so it pays its own neighbour-grid build):
+    g(r) 0.01 sigma to 5 sigma (400 bins), the contact histogram (5 bins of
+    0.01), and 10^7 uniform Widom insertions.  Beside them, one g(r) sample
+    of the oracle's restatement of compute_rdf (all pairs, compute_rdf.c:118-126)
+    on the same configuration, on one core."""
+    def per_sample(fn, reps):
+        total = 0.0
+        for _ in range(reps):
+            ctx.sweeps(1)
+            ctx.sync()
+            t0 = time.perf_counter()
+            fn()
+            total += time.perf_counter() - t0
+        return total / reps
+    out = {
+        "rdf_400_bins_ms": 1e3 * per_sample(lambda: ctx.rdf_counts(0.01, 400, 5.0), 20),
+        "contact_5_bins_ms": 1e3 * per_sample(lambda: ctx.contact_counts(0.01, 5, 1.05), 20),
+        "widom_1e7_uniform_ms": 1e3 * per_sample(lambda: ctx.widom_uniform(7, 0, 0, 10_000_000), 10),
+    }
+    if not no_cpu:
+        from oracle import oracle as O
+        o = O.System(2, 20, rho=0.85, neigh_dr=1.05, max_part=10, dr_max=0.1, seed=1)
+        o.set_config(ctx.download())
+        t0 = time.perf_counter()
+        o.rdf_counts(0.01, 400, 5.0)
+        out["cpu_rdf_400_bins_ms"] = 1e3 * (time.perf_counter() - t0)
+    return out
+
+
 def traffic_from_profile():
     path = os.path.join(ROOT, "profiles", "sweep_phase_n32000_pmc.json")
     if not os.path.exists(path):
@@ -179,6 +210,7 @@ def main() -> None:
                      "launches_timed": launches,
                      "algorithmic_bytes_per_launch": per_launch_bytes},
     }
+    line["observables"] = observables(ctx, args.no_cpu)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line))
