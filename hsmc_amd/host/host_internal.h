@@ -2,10 +2,13 @@
 #ifndef HSMC_HOST_INTERNAL_H
 #define HSMC_HOST_INTERNAL_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 /* exit with the reference's print-and-exit behaviour on a device error */
 void host_gpu_check(int st, const char *where);
+/* malloc that prints and exits when memory runs out */
+void *host_alloc(size_t bytes);
 /* send the host copy to the GPU if it was written since the last upload */
 void host_sync_device(void);
 /* note that the GPU moved particles (host copy and cell list are behind) */

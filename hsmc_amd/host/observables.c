@@ -27,8 +27,8 @@ static double *pressv_rr = NULL, *pressv_hist = NULL;
 
 static void pressv_hist_alloc(void) {   /* compute_press.c:85-95 */
   pressv_hist_nn = (int)((pressv_rmax - 1.0) / G_IN.pressv_dr);
-  pressv_rr = (double *)malloc(sizeof(double) * pressv_hist_nn);
-  pressv_hist = (double *)malloc(sizeof(double) * pressv_hist_nn);
+  pressv_rr = (double *)host_alloc(sizeof(double) * pressv_hist_nn);
+  pressv_hist = (double *)host_alloc(sizeof(double) * pressv_hist_nn);
   if (pressv_rr == NULL || pressv_hist == NULL) {
     printf("ERROR: Failed histogram allocation\n");
     exit(EXIT_FAILURE);
@@ -76,7 +76,7 @@ void compute_pressv(bool init) {
     }
   }
   host_sync_device();
-  int64_t *cnt = (int64_t *)malloc(sizeof(int64_t) * (pressv_hist_nn > 0 ? pressv_hist_nn : 1));
+  int64_t *cnt = (int64_t *)host_alloc(sizeof(int64_t) * (pressv_hist_nn > 0 ? pressv_hist_nn : 1));
   int64_t oor = 0;
   host_gpu_check(hsmc_gpu_contact_counts(hsmc_host_gpu(), 0, G_IN.pressv_dr, pressv_hist_nn,
                                          pressv_rmax, cnt, &oor),
@@ -115,8 +115,8 @@ void presst_hist_free(void) {
 void compute_presst(bool init) {
   if (init) {   /* compute_press.c:209-219 */
     presst_hist_nn = (int)(G_IN.presst_xi_max / G_IN.presst_dxi);
-    presst_xi = (double *)malloc(sizeof(double) * (presst_hist_nn > 0 ? presst_hist_nn : 1));
-    presst_hist = (double *)malloc(sizeof(double) * (presst_hist_nn > 0 ? presst_hist_nn : 1));
+    presst_xi = (double *)host_alloc(sizeof(double) * (presst_hist_nn > 0 ? presst_hist_nn : 1));
+    presst_hist = (double *)host_alloc(sizeof(double) * (presst_hist_nn > 0 ? presst_hist_nn : 1));
     if (presst_xi == NULL || presst_hist == NULL) {
       printf("ERROR: Failed histogram allocation\n");
       exit(EXIT_FAILURE);
@@ -181,7 +181,7 @@ void compute_mu(bool init) {
   const int n = G_IN.mu_insertions;
   if (G_IN.gpu_serial) {
     /* the reference's trial points from the MT stream (widom_rand_pos) */
-    double *q = (double *)malloc(sizeof(double) * 3 * (n > 0 ? n : 1));
+    double *q = (double *)host_alloc(sizeof(double) * 3 * (n > 0 ? n : 1));
     for (int i = 0; i < n; i++) {
       q[3 * i] = rng_get_double() * b.lx;
       q[3 * i + 1] = rng_get_double() * b.ly;
@@ -233,8 +233,8 @@ void compute_rdf(bool init, int sweep) {
              G_IN.rdf_rmax);
     }
     rdf_hist_nn = (int)((G_IN.rdf_rmax - 1.0) / G_IN.rdf_dr);
-    rdf_rr = (double *)malloc(sizeof(double) * rdf_hist_nn);
-    rdf_hist = (double *)malloc(sizeof(double) * rdf_hist_nn);
+    rdf_rr = (double *)host_alloc(sizeof(double) * rdf_hist_nn);
+    rdf_hist = (double *)host_alloc(sizeof(double) * rdf_hist_nn);
     if (rdf_rr == NULL || rdf_hist == NULL) {
       printf("ERROR: Failed histogram allocation\n");
       exit(EXIT_FAILURE);
@@ -246,7 +246,7 @@ void compute_rdf(bool init, int sweep) {
   }
   G_IN.rdf_rmax = G_IN.rdf_dr * rdf_hist_nn + 1.0;
   host_sync_device();
-  int64_t *cnt = (int64_t *)malloc(sizeof(int64_t) * rdf_hist_nn);
+  int64_t *cnt = (int64_t *)host_alloc(sizeof(int64_t) * rdf_hist_nn);
   int64_t oor = 0;
   host_gpu_check(hsmc_gpu_rdf_counts(hsmc_host_gpu(), 0, G_IN.rdf_dr, rdf_hist_nn, G_IN.rdf_rmax,
                                      cnt, &oor),

@@ -37,6 +37,15 @@ void host_gpu_check(int st, const char *where) {
   }
 }
 
+void *host_alloc(size_t bytes) {
+  void *p = malloc(bytes > 0 ? bytes : 1);
+  if (!p) {
+    printf("ERROR: out of host memory (%zu bytes)\n", bytes);
+    exit(EXIT_FAILURE);
+  }
+  return p;
+}
+
 /* sim_box_init (sim_info.c:32-71) */
 void sim_box_init(int cell_type, int nx, int ny, int nz, double rho) {
   int part_cell;
@@ -74,7 +83,7 @@ void part_alloc(void) {
   }
   int part_tot = sim_box.cell_x * sim_box.cell_y * sim_box.cell_z * part_cell;
   free(part_conf);
-  part_conf = malloc(part_tot * sizeof(*part_conf));
+  part_conf = host_alloc(part_tot * sizeof(*part_conf));
   if (part_conf == NULL) {
     printf("ERROR: Failed particle allocation\n");
     exit(EXIT_FAILURE);
@@ -264,8 +273,8 @@ static cl_info neigh_info;
 static int cells_alloc_rows = 0;
 
 void cell_list_alloc_arr(int ***arr, int rows, int cols) {
-  *arr = (int **)malloc(rows * sizeof(int *));
-  for (int i = 0; i < rows; i++) (*arr)[i] = (int *)malloc(cols * sizeof(int));
+  *arr = (int **)host_alloc(rows * sizeof(int *));
+  for (int i = 0; i < rows; i++) (*arr)[i] = (int *)host_alloc(cols * sizeof(int));
 }
 
 void cell_list_free_arr(int ***arr, int rows) {
@@ -345,7 +354,7 @@ void cell_list_new(void) {
   ensure_gpu();
   host_sync_device();
   const int C = neigh_info.num_tot, mp = G_IN.neigh_max_part;
-  int *pc = (int *)malloc(sizeof(int) * (size_t)C * mp);
+  int *pc = (int *)host_alloc(sizeof(int) * (size_t)C * mp);
   int st = hsmc_gpu_cell_list(gpu, 0, neigh_info.num_x, neigh_info.num_y, neigh_info.num_z, mp, pc,
                               NULL);
   if (st == HSMC_ERR_CELL_OVERFLOW) {
