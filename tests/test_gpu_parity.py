@@ -183,3 +183,62 @@ def test_full_size_sweeps_keep_hard_core():
     # contact pairs only at r >= 1: the histogram's bin of r < 1 is empty
     cnt, oor = ctx.contact_counts(0.01, 5, 1.05)
     assert oor == 0 and cnt.sum() > 0
+
+
+def test_full_size_observables_match_oracle():
+    """At the benchmark's own size (N = 32000, rho = 0.85, after 50 sweeps) the
+    GPU observables equal the oracle's restatement of the reference on the same
+    configuration, bit for bit: g(r) counts (all pairs, compute_rdf.c:110-128),
+    contact counts (compute_press.c:123-165), presst flags
+    (compute_press.c:243-273), pair distances and Widom hits
+    (compute_widom_chem_pot.c:82-120)."""
+    conf, box = gpu.lattice(2, 20, 0.85)
+    ctx = gpu.Context([dict(n=len(conf), box=box, dr_max=0.1, seed=4)])
+    ctx.upload(conf)
+    ctx.sweeps(50)
+    cfg = ctx.download()
+    o = O.System(2, 20, rho=0.85, neigh_dr=1.05, max_part=10)
+    o.set_config(cfg)
+    L = box[0]
+    g_cnt, g_oor = ctx.rdf_counts(0.01, 400, 5.0)
+    o_cnt, o_oor = o.rdf_counts(0.01, 400, 5.0)
+    assert np.array_equal(g_cnt, o_cnt) and g_oor == o_oor
+    g_c, _ = ctx.contact_counts(0.01, 5, 1.05)
+    o_c, _ = o.contact_counts(0.01, 5, 1.05)
+    assert np.array_equal(g_c, o_c)
+    sf = [(1.0 - (k + 1) * 0.0001) ** (1.0 / 3.0) for k in range(20)]
+    assert np.array_equal(ctx.any_overlap_scaled(sf), o.presst_flags(sf))
+    rng = np.random.default_rng(8)
+    i = rng.integers(0, 32000, 2000).astype(np.int32)
+    j = rng.integers(0, 32000, 2000).astype(np.int32)
+    assert np.array_equal(ctx.pair_dist(i, j), [o.compute_dist(a, b) for a, b in zip(i, j)])
+    pts = rng.random((200000, 3)) * L
+    assert np.array_equal(ctx.widom_points(pts), o.widom_points(pts))
+    ctx.close()
+
+
+def test_widom_hits_match_oracle_at_n108000():
+    """N = 108000 at rho = 0.90 (the learned-proposal benchmark's system): the
+    decisions for 3 * 10^5 uniform trial points (almost all overlap at this
+    density) and for points 1 -/+ 1e-12 from sphere centres (decided by the
+    last bits of the FP64 distance) equal the reference algorithm's."""
+    conf, box = gpu.lattice(2, 30, 0.90)
+    ctx = gpu.Context([dict(n=len(conf), box=box, dr_max=0.1, seed=6)])
+    ctx.upload(conf)
+    ctx.sweeps(30)
+    cfg = ctx.download()
+    o = O.System(2, 30, rho=0.90, neigh_dr=1.0, max_part=10)
+    o.set_config(cfg)
+    rng = np.random.default_rng(9)
+    L = box[0]
+    pts = rng.random((300000, 3)) * L
+    assert np.array_equal(ctx.widom_points(pts), o.widom_points(pts))
+    # points at distance 1 -/+ 1e-12 from 2000 centres, kept inside [0, L)
+    c = cfg[rng.integers(0, len(cfg), 2000), 1:]
+    u = rng.normal(size=(2000, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    edge = np.concatenate([c + u * (1.0 - 1e-12), c + u * (1.0 + 1e-12)]) % L
+    got = ctx.widom_points(edge)
+    assert np.array_equal(got, o.widom_points(edge))
+    assert got[:2000].sum() == 0   # inside a sphere: never insertable
+    ctx.close()
