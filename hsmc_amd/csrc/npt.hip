@@ -41,6 +41,10 @@ constexpr uint32_t kNptTag = 0x4E505400u;  // "NPT\0"
 
 // Any pair of a replica with a proposed compression (sf[r] > 0) closer than 1
 // after scaling: thread per sphere, neighbours from the last sweep's cells.
+// Each pair is tested once: from the sphere's own cell towards the 13
+// lexicographically forward neighbour cells, and within the cell towards
+// higher ids (the scaled distance is the same from either end, as
+// fl(a - b) = -fl(b - a)).  A thread stops once its replica has an overlap.
 __global__ void __launch_bounds__(256)
 k_vol_overlap(const RepDev *__restrict__ reps, const int *__restrict__ prep, int P, int C,
               const int *__restrict__ cnt, const int *__restrict__ spid,
@@ -54,6 +58,7 @@ k_vol_overlap(const RepDev *__restrict__ reps, const int *__restrict__ prep, int
   const int r = prep[i];
   const double sf = sfr[r];
   if (!(sf > 0.0)) return;
+  if (__hip_atomic_load(&flags[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const RepDev rp = reps[r];
   const unsigned long long sweep = *sweep_next - 1ull;
   const int *cntb = cnt + (size_t)(sweep & 1ull) * C;
@@ -68,16 +73,18 @@ k_vol_overlap(const RepDev *__restrict__ reps, const int *__restrict__ prep, int
   const int cz = shifted_cell(az, shz, rp.lz, rp.iz, rp.gz);
   const int me = i - rp.poff;
   bool ov = false;
-  for (int d = 0; d < 27 && !ov; ++d) {
+  // d = 13 is the own cell; d = 14..26 are the forward offsets
+  for (int d = 13; d < 27 && !ov; ++d) {
     const int dx = d / 9 - 1, dy = (d / 3) % 3 - 1, dz = d % 3 - 1;
     if (!stencil_axis_ok(dx, rp.gx) || !stencil_axis_ok(dy, rp.gy) || !stencil_axis_ok(dz, rp.gz))
       continue;
+    if (d > 13 && __hip_atomic_load(&flags[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
     const int c = rp.coff + (wrap_index(cx + dx, rp.gx) * rp.gy + wrap_index(cy + dy, rp.gy)) * rp.gz +
                   wrap_index(cz + dz, rp.gz);
     const int n = min(cntb[c], cap);
     for (int q = 0; q < n; ++q) {
       const size_t s = (size_t)c * cap + q;
-      if (spid[s] == me) continue;
+      if (d == 13 && spid[s] <= me) continue;  // own cell: each pair once, self excluded
       // compute_dist with sf (moves.c:400-431), reference particle first
       if (dist2_scaled(ax, ay, az, sx[s], sy[s], sz[s], rp.lx, rp.ly, rp.lz, sf) < 1.0) {
         ov = true;
