@@ -410,32 +410,17 @@ k_training_rows(NGrid g, Vox vx, uint32_t k0, uint32_t k1, unsigned long long sa
   for (int k = 0; k < NIN; ++k) X[(size_t)i * NIN + k] = d[k];
 }
 
-// Scores of all voxels: descriptor + network, fused (descriptors never leave
-// the chip).  Block = 4 waves = 256 voxels; per-block max and sum exp(f - max).
-__global__ void __launch_bounds__(256)
-k_voxel_scores(NGrid g, Vox vx, float far2, const MlpDev *__restrict__ net,
-               float *__restrict__ f, float *__restrict__ bmax, float *__restrict__ bsum) {
-  __shared__ MlpDev s;
-  __shared__ uint16_t X[4][64 * KIN];
-  __shared__ uint16_t H1[4][64 * NH];
-  __shared__ float ys[4][64];
-  __shared__ float red[8];
-  load_net(net, &s);
+// Network scores of a block's 256 voxels from their descriptors (one wave per
+// 64 voxels, MFMA) and the block's max and sum exp(f - max).
+__device__ void score_block(const MlpDev *s, uint16_t (*X)[64 * KIN], uint16_t (*H1)[64 * NH],
+                            float (*ys)[64], float *red, const float d[NIN], bool ok, long v,
+                            float *__restrict__ f, float *__restrict__ bmax,
+                            float *__restrict__ bsum) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const long v = (long)blockIdx.x * 256 + threadIdx.x;
-  float d[NIN];
-  if (v < vx.V) {
-    double cx, cy, cz;
-    voxel_centre(vx, v, cx, cy, cz);
-    descriptor(g, cx, cy, cz, far2, d);
-  } else {
-    for (int k = 0; k < NIN; ++k) d[k] = 0.f;
-  }
   for (int k = 0; k < KIN; ++k) X[w][lane * KIN + k] = k < NIN ? f2bf(d[k]) : (uint16_t)0;
   wave_sync();
-  mlp_forward_wave(&s, X[w], H1[w], nullptr, ys[w]);
+  mlp_forward_wave(s, X[w], H1[w], nullptr, ys[w]);
   const float val = ys[w][lane];
-  const bool ok = v < vx.V;
   if (ok) f[v] = val;
   float m = ok ? val : -INFINITY;
   for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
@@ -451,6 +436,131 @@ k_voxel_scores(NGrid g, Vox vx, float far2, const MlpDev *__restrict__ net,
     bmax[blockIdx.x] = bm;
     bsum[blockIdx.x] = red[4] + red[5] + red[6] + red[7];
   }
+}
+
+// Scores of all voxels: descriptor + network, fused (descriptors never leave
+// the chip).  Block = 256 consecutive voxels, each searching the grid alone.
+__global__ void __launch_bounds__(256)
+k_voxel_scores(NGrid g, Vox vx, float far2, const MlpDev *__restrict__ net,
+               float *__restrict__ f, float *__restrict__ bmax, float *__restrict__ bsum) {
+  __shared__ MlpDev s;
+  __shared__ uint16_t X[4][64 * KIN];
+  __shared__ uint16_t H1[4][64 * NH];
+  __shared__ float ys[4][64];
+  __shared__ float red[8];
+  load_net(net, &s);
+  const long v = (long)blockIdx.x * 256 + threadIdx.x;
+  float d[NIN];
+  if (v < vx.V) {
+    double cx, cy, cz;
+    voxel_centre(vx, v, cx, cy, cz);
+    descriptor(g, cx, cy, cz, far2, d);
+  } else {
+    for (int k = 0; k < NIN; ++k) d[k] = 0.f;
+  }
+  score_block(&s, X, H1, ys, red, d, v < vx.V, v, f, bmax, bsum);
+}
+
+// The same over 4 x 8 x 8 voxel tiles (one wave per x layer): the block first
+// gathers into LDS, as FP32 offsets from the tile centre, every sphere whose
+// minimum-image offset lies within the tile's half extent + smin on each
+// axis (a superset of the spheres within smin of any of its voxel centres),
+// then each voxel takes its 12 nearest from LDS.  The host uses it only when
+// every box edge exceeds 2 (tile extent + smin), so one image serves the
+// whole tile.  A tile whose set would overflow the LDS list (impossible
+// below close packing: a 4 x 5 x 5 sigma^3 box holds at most ~254 spheres)
+// searches the grid per voxel instead.
+constexpr int TX = 4, TY = 8, TZ = 8, TCAP = 320;
+
+__global__ void __launch_bounds__(256)
+k_voxel_scores_tiled(NGrid g, Vox vx, float far2, double smin, const MlpDev *__restrict__ net,
+                     float *__restrict__ f, float *__restrict__ bmax, float *__restrict__ bsum) {
+  __shared__ MlpDev s;
+  __shared__ uint16_t X[4][64 * KIN];
+  __shared__ uint16_t H1[4][64 * NH];
+  __shared__ float ys[4][64];
+  __shared__ float red[8];
+  __shared__ float qx[TCAP], qy[TCAP], qz[TCAP];
+  __shared__ int ncand;
+  if (threadIdx.x == 0) ncand = 0;
+  load_net(net, &s);  // ends with a block barrier
+  const int ntz = (vx.mz + TZ - 1) / TZ, nty = (vx.my + TY - 1) / TY;
+  const int bz = blockIdx.x % ntz, by = (blockIdx.x / ntz) % nty, bx = blockIdx.x / (ntz * nty);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ix = bx * TX + w, iy = by * TY + (lane >> 3), iz = bz * TZ + (lane & 7);
+  const bool ok = ix < vx.mx && iy < vx.my && iz < vx.mz;
+  // tile centre and the half extent of the gathered box
+  const double ccx = (bx * TX + TX * 0.5) * vx.vx, ccy = (by * TY + TY * 0.5) * vx.vy,
+               ccz = (bz * TZ + TZ * 0.5) * vx.vz;
+  const double ex = (TX - 1) * 0.5 * vx.vx + smin, ey = (TY - 1) * 0.5 * vx.vy + smin,
+               ez = (TZ - 1) * 0.5 * vx.vz + smin;
+  // grid cells overlapping the box, per axis (all cells when it spans them)
+  int lo[3], nc[3];
+  const double cc[3] = {ccx, ccy, ccz}, ee[3] = {ex, ey, ez}, side[3] = {g.sx, g.sy, g.sz};
+  const int gn[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int l = (int)floor((cc[a] - ee[a]) / side[a]);
+    const int h = (int)floor((cc[a] + ee[a]) / side[a]);
+    nc[a] = h - l + 1;
+    lo[a] = l;
+    if (nc[a] >= gn[a]) {
+      nc[a] = gn[a];
+      lo[a] = 0;
+    }
+  }
+  const int ncell = nc[0] * nc[1] * nc[2];
+  for (int k = threadIdx.x; k < ncell; k += blockDim.x) {
+    const int kx = k / (nc[1] * nc[2]), ky = (k / nc[2]) % nc[1], kz = k % nc[2];
+    const int c = (wrap_index(lo[0] + kx, g.nx) * g.ny + wrap_index(lo[1] + ky, g.ny)) * g.nz +
+                  wrap_index(lo[2] + kz, g.nz);
+    for (int p = g.start[c]; p < g.start[c + 1]; ++p) {
+      const double dx = min_image(g.px[p] - ccx, g.lx, g.hx);
+      const double dy = min_image(g.py[p] - ccy, g.ly, g.hy);
+      const double dz = min_image(g.pz[p] - ccz, g.lz, g.hz);
+      if (fabs(dx) <= ex && fabs(dy) <= ey && fabs(dz) <= ez) {
+        const int slot = atomicAdd(&ncand, 1);
+        if (slot < TCAP) {
+          qx[slot] = (float)dx;
+          qy[slot] = (float)dy;
+          qz[slot] = (float)dz;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int n = ncand;
+  float d[NIN];
+  if (ok) {
+    const double vxc = (ix + 0.5) * vx.vx, vyc = (iy + 0.5) * vx.vy, vzc = (iz + 0.5) * vx.vz;
+    if (n <= TCAP) {
+      float best[NIN];
+#pragma unroll
+      for (int k = 0; k < NIN; ++k) best[k] = far2;
+      const float ox = (float)(vxc - ccx), oy = (float)(vyc - ccy), oz = (float)(vzc - ccz);
+      for (int q = 0; q < n; ++q) {
+        const float ax = qx[q] - ox, ay = qy[q] - oy, az = qz[q] - oz;
+        const float d2 = ax * ax + ay * ay + az * az;
+        if (d2 < best[NIN - 1]) {
+          best[NIN - 1] = d2;
+#pragma unroll
+          for (int k = NIN - 1; k > 0; --k) {
+            const float l2 = fminf(best[k - 1], best[k]), h2 = fmaxf(best[k - 1], best[k]);
+            best[k - 1] = l2;
+            best[k] = h2;
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NIN; ++k) d[k] = sqrtf(best[k]) - 1.0f;
+    } else {
+      descriptor(g, vxc, vyc, vzc, far2, d);
+    }
+  } else {
+    for (int k = 0; k < NIN; ++k) d[k] = 0.f;
+  }
+  const long v = ((long)ix * vx.my + iy) * vx.mz + iz;
+  score_block(&s, X, H1, ys, red, d, ok, v, f, bmax, bsum);
 }
 
 // Proposal mass of voxel v: q_v = mix softmax(f)_v + (1 - mix) / V.
@@ -740,6 +850,31 @@ float far2_of(const NGrid &g) {
   return (float)(s * s);
 }
 
+// Voxel scores into f, and per-block max / sum exp(f - max) into the scratch
+// returned in *bred (*nblk blocks): the tiled kernel when every box edge
+// exceeds 2 (tile extent + smin), else the per-voxel one.
+int launch_scores(hsmc_gpu_ctx *ctx, const NGrid &g, const Vox &vx, const MlpDev *net, float *f,
+                  float **bred, long *nblk) {
+  const double smin = std::min(g.sx, std::min(g.sy, g.sz));
+  const bool tiled = g.lx > 2.0 * ((TX - 1) * vx.vx + smin) * (1.0 + 1e-9) &&
+                     g.ly > 2.0 * ((TY - 1) * vx.vy + smin) * (1.0 + 1e-9) &&
+                     g.lz > 2.0 * ((TZ - 1) * vx.vz + smin) * (1.0 + 1e-9);
+  const long nb = tiled ? (long)((vx.mx + TX - 1) / TX) * ((vx.my + TY - 1) / TY) * ((vx.mz + TZ - 1) / TZ)
+                        : (vx.V + 255) / 256;
+  float *b = (float *)scratch(ctx, L_FRED, sizeof(float) * 2 * (size_t)nb);
+  if (!b) return HSMC_ERR_NO_MEMORY;
+  if (tiled)
+    hipLaunchKernelGGL(k_voxel_scores_tiled, dim3((unsigned)nb), dim3(256), 0, ctx->stream, g, vx,
+                       far2_of(g), smin, net, f, b, b + nb);
+  else
+    hipLaunchKernelGGL(k_voxel_scores, dim3((unsigned)nb), dim3(256), 0, ctx->stream, g, vx,
+                       far2_of(g), net, f, b, b + nb);
+  HSMC_HIP(ctx, hipGetLastError());
+  *bred = b;
+  *nblk = nb;
+  return HSMC_OK;
+}
+
 
 }  // namespace
 
@@ -923,20 +1058,19 @@ int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mi
   if (st) return st;
   const Vox vx = make_vox(ctx->reps[replica], voxel);
   if (vx.V > (1L << 31) - 1) return HSMC_ERR_NOT_SUPPORTED;
-  const long nblk = (vx.V + 255) / 256;
   const long ncdf = (vx.V + 4095) / 4096;
   float *f = (float *)scratch(ctx, L_F, sizeof(float) * (size_t)vx.V);
-  float *bred = (float *)scratch(ctx, L_FRED, sizeof(float) * 2 * (size_t)nblk);
   double *cdf = (double *)scratch(ctx, L_CDF, sizeof(double) * (size_t)vx.V);
   double *bsum = (double *)scratch(ctx, L_BSUM, sizeof(double) * (size_t)(ncdf + 1));
   double *acc = (double *)scratch(ctx, L_ACC, sizeof(double) * 2);
-  if (!f || !bred || !cdf || !bsum || !acc) return HSMC_ERR_NO_MEMORY;
+  if (!f || !cdf || !bsum || !acc) return HSMC_ERR_NO_MEMORY;
   const bool scores_ok = m->sc_replica == replica && m->sc_version == ctx->version &&
                          m->sc_step == m->step && m->sc_voxel == voxel;
   if (!scores_ok) {
-    hipLaunchKernelGGL(k_voxel_scores, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, g, vx,
-                       far2_of(g), m->net, f, bred, bred + nblk);
-    HSMC_HIP(ctx, hipGetLastError());
+    float *bred = nullptr;
+    long nblk = 0;
+    st = launch_scores(ctx, g, vx, m->net, f, &bred, &nblk);
+    if (st) return st;
     std::vector<float> hb(2 * nblk);
     HSMC_HIP(ctx, hipMemcpyAsync(hb.data(), bred, sizeof(float) * 2 * nblk, hipMemcpyDeviceToHost, ctx->stream));
     HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -998,14 +1132,13 @@ int hsmc_gpu_voxel_scores(hsmc_gpu_mlp *m, int replica, double voxel, float *sco
   NGrid g;
   int st = build_ngrid(ctx, replica, kDescCut, L_GRID_A, g);
   if (st) return st;
-  const long nblk = (vx.V + 255) / 256;
   float *f = (float *)scratch(ctx, L_F, sizeof(float) * (size_t)vx.V);
-  float *bred = (float *)scratch(ctx, L_FRED, sizeof(float) * 2 * (size_t)nblk);
-  if (!f || !bred) return HSMC_ERR_NO_MEMORY;
+  if (!f) return HSMC_ERR_NO_MEMORY;
   m->sc_replica = -1;
-  hipLaunchKernelGGL(k_voxel_scores, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, g, vx,
-                     far2_of(g), m->net, f, bred, bred + nblk);
-  HSMC_HIP(ctx, hipGetLastError());
+  float *bred = nullptr;
+  long nblk = 0;
+  st = launch_scores(ctx, g, vx, m->net, f, &bred, &nblk);
+  if (st) return st;
   HSMC_HIP(ctx, hipMemcpyAsync(scores, f, sizeof(float) * vx.V, hipMemcpyDeviceToHost, ctx->stream));
   HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return HSMC_OK;

@@ -120,3 +120,27 @@ def test_systematic_sampler_weights():
     # with a proposal that is not uniform the count is still exactly n
     r = net.widom(0.25, 0.9, 5, 2, 54321)
     assert r["n"] == 54321 and r["hits"] > 0
+
+
+@pytest.mark.parametrize("n,rho", [(10, 0.8), (3, 0.6)])
+def test_voxel_scores_match_point_path(n, rho):
+    """Voxel scores (tiled kernel for the N = 4000 box of edge 17.1, per-voxel
+    kernel for the N = 108 box of edge 5.65, below the 6.5 tiles need) equal the network
+    applied to descriptors taken at the voxel centres by the per-point FP64
+    path, up to the rounding of FP32 distances feeding BF16 inputs."""
+    ctx, L = _ctx_with(n, rho, 100)
+    net = gpu.Mlp(ctx, seed=4)
+    net.train(0.25, 5, 0, 100000, epochs=2, batch=4096, lr=2e-3, beta=10.0)
+    scores = net.voxel_scores(0.25)
+    m = int(math.ceil(L / 0.25))
+    assert len(scores) == m ** 3
+    v = np.random.default_rng(2).choice(m ** 3, size=min(5000, m ** 3), replace=False)
+    side = L / m
+    ix, iy, iz = v // (m * m), (v // m) % m, v % m
+    centres = np.stack([(ix + 0.5) * side, (iy + 0.5) * side, (iz + 0.5) * side], axis=1)
+    ref = net.forward(gpu.point_descriptors(ctx, centres))
+    diff = np.abs(scores[v] - ref)
+    scale = max(1.0, float(np.max(np.abs(ref))))
+    assert np.quantile(diff, 0.99) <= 1e-4 * scale, np.quantile(diff, 0.99)
+    assert diff.max() <= 0.05 * scale, diff.max()
+    ctx.close()
