@@ -112,11 +112,14 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=None,
                     help="untimed sweeps first (default: 1000, 200, 5 for the same workloads)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["n32000", "batched", "widom", "npt"], default="n32000",
+    ap.add_argument("--workload", choices=["n32000", "batched", "widom", "npt", "widom_npt"],
+                    default="n32000",
                     help="n32000: the headline single system; batched: 256 replicas "
                          "N = 4 n^3, n in 3..15, rho in [0.30, 0.93]; widom: GPU seconds to "
                          "sigma(beta mu_ex) = 0.01 at N = 108000, rho = 0.90, learned vs uniform; "
-                         "npt: 64 NpT replicas of N = 256000 at beta P in {5, 8, 11}")
+                         "npt: 64 NpT replicas of N = 256000 at beta P in {5, 8, 11}; "
+                         "widom_npt: learned vs uniform Widom on NpT configurations of N = 256000 "
+                         "at beta P in {5, 8, 11} (--warmup NVT melt sweeps, --steps NpT sweeps)")
     ap.add_argument("--sigma", type=float, default=0.01, help="widom: target sigma(beta mu_ex)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", type=str, default="",
@@ -127,7 +130,8 @@ def main() -> None:
                     help="only run one device-to-device copy of this many MiB (torch) and exit: "
                          "the known-size reference for rocprofv3 FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
-    defaults = {"n32000": (20000, 1000), "batched": (2000, 200), "widom": (0, 0), "npt": (20, 5)}
+    defaults = {"n32000": (20000, 1000), "batched": (2000, 200), "widom": (0, 0), "npt": (20, 5),
+                "widom_npt": (2000, 20000)}
     if args.steps is None:
         args.steps = defaults[args.workload][0]
     if args.warmup is None:
@@ -153,6 +157,8 @@ def main() -> None:
         return widom(args)
     if args.workload == "npt":
         return npt(args)
+    if args.workload == "widom_npt":
+        return widom_npt(args)
 
     from hsmc_amd import gpu
     conf, box = gpu.lattice(2, 20, 0.85)
@@ -266,29 +272,20 @@ def npt(args) -> None:
     volume moves, each checked by an all-pairs scaled-overlap reduction)."""
     import numpy as np
 
-    from hsmc_amd import analysis as A
     from hsmc_amd import gpu
-
-    def cs_density(p):
-        lo, hi = 0.1, 0.99
-        for _ in range(100):
-            mid = 0.5 * (lo + hi)
-            if mid * A.carnahan_starling(mid)["Z"] < p:
-                lo = mid
-            else:
-                hi = mid
-        return 0.5 * (lo + hi)
 
     levels = [5.0, 8.0, 11.0]
     press = [levels[0]] * 21 + [levels[1]] * 21 + [levels[2]] * 22
     R, n_cells, dv = len(press), 40, 0.001
     reps, confs = [], []
     for r, p in enumerate(press):
-        conf, box = gpu.lattice(2, n_cells, cs_density(p))
+        conf, box = gpu.lattice(2, n_cells, _cs_density(p))
         reps.append(dict(n=len(conf), box=box, dr_max=0.1, seed=1 + r))
         confs.append(conf)
     total = sum(r["n"] for r in reps)
     ctx = gpu.Context(reps)
+    if args.tuning:
+        ctx.set_tuning(*(int(v) for v in args.tuning.split(",")))
     for r, c in enumerate(confs):
         ctx.upload(c, r)
     del confs
@@ -309,7 +306,7 @@ def npt(args) -> None:
     for p in levels:
         rs = [r for r in range(R) if press[r] == p]
         vals = [reps[r]["n"] / float(np.prod(ctx.box(r))) for r in rs]
-        dens[str(p)] = {"mean_density": float(np.mean(vals)), "carnahan_starling": cs_density(p)}
+        dens[str(p)] = {"mean_density": float(np.mean(vals)), "carnahan_starling": _cs_density(p)}
     print(json.dumps({
         "metric": "MC move attempts/s (64 NpT replicas, N=256000, beta P in {5,8,11})",
         "value": attempts / secs, "unit": "MC move attempts/s", "n_gpus": 1, "steps": args.steps,
@@ -334,39 +331,25 @@ def _timed(fn):
     return out, time.perf_counter() - t0
 
 
-def _widom_density(args, rho, n_uniform, n_learned):
+def _compare_widom(args, systems, n_uniform, n_learned, label):
     """GPU seconds of insertion work until sigma(beta mu_ex) reaches the
-    target at density rho, N = 108000 (fcc 4*30^3, melted by 10^4 sweeps),
-    plain uniform insertions against the learned proposal on the same K = 4
-    configurations (10^3 sweeps apart, one context each).  The network is
-    trained once on 10^6 uniform insertions of the first configuration (its
-    time is reported apart; the learned seconds include scoring every
-    configuration's voxels)."""
+    target, plain uniform insertions against the learned proposal, on the
+    same K configurations (one context each; systems = [(conf, box), ...]).
+    The network is trained once on 10^6 uniform insertions of the first
+    configuration (its time is reported apart; the learned seconds include
+    scoring every configuration's voxels)."""
     import math
 
     import numpy as np
 
-    from hsmc_amd import analysis as A
     from hsmc_amd import gpu
-    n_cells, K = 30, 4
+    K = len(systems)
     voxel, mix = 0.25, 0.9
-    conf, box = gpu.lattice(2, n_cells, rho)
-    N = len(conf)
-    base = gpu.Context([dict(n=N, box=box, dr_max=0.1, seed=11)])
-    base.upload(conf)
-    base.sweeps(10000)   # the superheated fcc crystal melts (mean square displacement grows linearly)
-    confs = []
-    for k in range(K):
-        base.sweeps(1000)
-        confs.append(base.download())
-    base.close()
     ctxs = []
-    for k in range(K):
-        c = gpu.Context([dict(n=N, box=box, dr_max=0.1, seed=100 + k)])
-        c.upload(confs[k])
+    for k, (conf, box) in enumerate(systems):
+        c = gpu.Context([dict(n=len(conf), box=box, dr_max=0.1, seed=100 + k)])
+        c.upload(conf)
         ctxs.append(c)
-    print(f"[bench] widom: {K} configurations of N={N} at rho={rho}", file=sys.stderr, flush=True)
-
     net0 = gpu.Mlp(ctxs[0], seed=5)
     loss, t_train = _timed(lambda: net0.train(voxel, 777, 0, 1_000_000, epochs=10, batch=4096,
                                                lr=2e-3, beta=20.0))
@@ -374,7 +357,7 @@ def _widom_density(args, rho, n_uniform, n_learned):
     nets = [net0] + [gpu.Mlp(c, seed=5) for c in ctxs[1:]]
     for m in nets[1:]:
         m.set(params)
-    print(f"[bench] trained: loss {loss:.4f} in {t_train:.3f} s", file=sys.stderr, flush=True)
+    print(f"[bench] {label}: trained, loss {loss:.4f} in {t_train:.3f} s", file=sys.stderr, flush=True)
 
     # at least 16 rounds per configuration, so the variance estimates behind
     # sigma (and the agreement test) are not themselves too noisy
@@ -410,19 +393,40 @@ def _widom_density(args, rho, n_uniform, n_learned):
                             insertions_per_s=n_total / secs)
 
     uni = run("uniform", n_uniform)
-    print(f"[bench] rho {rho} uniform: {uni}", file=sys.stderr, flush=True)
+    print(f"[bench] {label} uniform: {uni}", file=sys.stderr, flush=True)
     lrn = run("learned", n_learned)
-    print(f"[bench] rho {rho} learned: {lrn}", file=sys.stderr, flush=True)
+    print(f"[bench] {label} learned: {lrn}", file=sys.stderr, flush=True)
     for c in ctxs:
         c.close()
     return {
-        "rho": rho,
         "ratio": uni["seconds"] / lrn["seconds"],
         "ratio_at_target": uni["seconds_at_target"] / lrn["seconds_at_target"],
         "agreement_sigmas": abs(uni["beta_mu"] - lrn["beta_mu"]) / math.hypot(uni["sigma"], lrn["sigma"]),
-        "carnahan_starling_beta_mu_ex": A.carnahan_starling(rho)["mu_ex"],
         "uniform": uni, "learned": lrn, "learned_training_seconds": t_train,
     }
+
+
+def _widom_density(args, rho, n_uniform, n_learned):
+    """Learned against uniform Widom at density rho, N = 108000 (fcc 4*30^3,
+    melted by 10^4 sweeps), on K = 4 configurations 10^3 sweeps apart."""
+    from hsmc_amd import analysis as A
+    from hsmc_amd import gpu
+    conf, box = gpu.lattice(2, 30, rho)
+    base = gpu.Context([dict(n=len(conf), box=box, dr_max=0.1, seed=11)])
+    if args.tuning:
+        base.set_tuning(*(int(v) for v in args.tuning.split(",")))
+    base.upload(conf)
+    base.sweeps(10000)   # the superheated fcc crystal melts (mean square displacement grows linearly)
+    systems = []
+    for k in range(4):
+        base.sweeps(1000)
+        systems.append((base.download(), box))
+    base.close()
+    print(f"[bench] widom: 4 configurations of N={len(conf)} at rho={rho}", file=sys.stderr, flush=True)
+    out = _compare_widom(args, systems, n_uniform, n_learned, f"rho {rho}")
+    out["rho"] = rho
+    out["carnahan_starling_beta_mu_ex"] = A.carnahan_starling(rho)["mu_ex"]
+    return out
 
 
 def widom(args) -> None:
@@ -451,6 +455,101 @@ def widom(args) -> None:
                          "seconds_uniform": v["uniform"]["seconds"], "seconds_learned": v["learned"]["seconds"]}
                       for k, v in res.items()},
     }))
+
+def _cs_density(p):
+    """Carnahan-Starling density at reduced pressure beta P sigma^3 = p."""
+    from hsmc_amd import analysis as A
+    lo, hi = 0.1, 0.99
+    for _ in range(100):
+        mid = 0.5 * (lo + hi)
+        if mid * A.carnahan_starling(mid)["Z"] < p:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def widom_npt(args) -> None:
+    """Learned against uniform Widom on NpT configurations of N = 256000 at
+    beta P = 5, 8, 11.  Per pressure, a batched context of 4 replicas starts
+    from fcc 4*40^3 at the Carnahan-Starling density of that pressure, melts
+    in NVT sweeps (--warmup; the crystal at rho 0.93 needs about 1.6e4 to
+    start diffusing), then runs NpT sweeps (--steps, ln-volume step 0.001);
+    its 4 final configurations (and boxes) are the Widom systems.  The
+    headline value is beta P = 11.  At N = 256000 an ln-volume step of up to
+    5e-4 compresses hundreds of contact pairs into overlap, so about 0.1% of
+    volume moves are accepted and the density stays near its start."""
+    import numpy as np
+
+    from hsmc_amd import analysis as A
+    from hsmc_amd import gpu
+    levels, K, n_cells, dv = [5.0, 8.0, 11.0], 4, 40, 0.001
+    melt, settle = args.warmup, args.steps
+    systems, vol, msd = {}, {}, {}
+    for p in levels:
+        conf, box = gpu.lattice(2, n_cells, _cs_density(p))
+        ctx = gpu.Context([dict(n=len(conf), box=box, dr_max=0.1, seed=500 + 10 * int(p) + k)
+                           for k in range(K)])
+        if args.tuning:
+            ctx.set_tuning(*(int(v) for v in args.tuning.split(",")))
+        for k in range(K):
+            ctx.upload(conf, k)
+        ctx.sweeps(melt)
+        vol[p] = ctx.npt_sweeps(settle, [p] * K, dv)
+        if not all(ctx.any_overlap_scaled_batch(1.0) == 1):
+            raise SystemExit("overlap found after the NpT sweeps")
+        # mean square displacement from the lattice, minimum image per axis
+        # (a lower bound once spheres travel beyond L/2); the crystal's
+        # vibrations alone give about 0.01
+        m = []
+        for k in range(K):
+            d = ctx.download(k)[:, 1:] - conf[:, 1:]
+            L = np.array(ctx.box(k))
+            d -= L * np.round(d / L)
+            m.append(float((d * d).sum(1).mean()))
+        msd[p] = float(np.mean(m))
+        systems[p] = [(ctx.download(k), tuple(ctx.box(k))) for k in range(K)]
+        dens = [len(conf) / float(np.prod(b)) for _, b in systems[p]]
+        print(f"[bench] widom_npt: beta P {p}: density {np.mean(dens):.5f} (start {_cs_density(p):.5f}), "
+              f"msd {msd[p]:.3f}, volume moves {int(vol[p][:, 1].sum())} of {int(vol[p][:, 0].sum())} "
+              f"accepted", file=sys.stderr, flush=True)
+        ctx.close()
+    sizes = {5.0: (1_000_000, 100_000), 8.0: (25_000_000, 250_000), 11.0: (200_000_000, 500_000)}
+    res = {}
+    for p in levels:
+        out = _compare_widom(args, systems[p], *sizes[p], f"beta P {p}")
+        dens = [len(c) / float(np.prod(b)) for c, b in systems[p]]
+        out["beta_P"] = p
+        out["density"] = float(np.mean(dens))
+        out["volume_moves_accepted"] = f"{int(vol[p][:, 1].sum())} of {int(vol[p][:, 0].sum())}"
+        out["msd_from_lattice"] = msd[p]
+        out["carnahan_starling_density"] = _cs_density(p)
+        out["carnahan_starling_beta_mu_ex"] = A.carnahan_starling(float(np.mean(dens)))["mu_ex"]
+        res[str(p)] = out
+    top = res["11.0"]
+    print(json.dumps({
+        "metric": "GPU seconds to sigma(beta mu_ex)=0.01 on NpT configurations at beta P = 11, "
+                  "learned vs uniform Widom",
+        "value": top["ratio"], "unit": "x less GPU time (uniform / learned)",
+        "n_gpus": 1, "higher_is_better": True, "vs_baseline": None, "dtype": "f64 (overlap), bf16 (network)",
+        "data": f"synthetic (fcc 4*40^3 at the Carnahan-Starling density of each pressure, {melt} NVT "
+                f"sweeps, then {settle} NpT sweeps per replica)",
+        "config": {"workload": "N=256000 NpT configurations, 4 replicas (one batched context) at each "
+                               "beta P in {5, 8, 11}, "
+                               "voxel 0.25 sigma, mix 0.9, network 12-64-64-1 trained on 1e6 uniform "
+                               "insertions per pressure",
+                   "target_sigma": args.sigma, "melt_sweeps": melt, "npt_sweeps": settle},
+        "ratio_at_target": top["ratio_at_target"], "agreement_sigmas": top["agreement_sigmas"],
+        "pressures": {k: {kk: v[kk] for kk in ("ratio", "ratio_at_target", "agreement_sigmas", "density",
+                                                "carnahan_starling_density", "carnahan_starling_beta_mu_ex",
+                                                "learned_training_seconds", "volume_moves_accepted",
+                                                "msd_from_lattice")}
+                      | {"beta_mu_uniform": v["uniform"]["beta_mu"], "beta_mu_learned": v["learned"]["beta_mu"],
+                         "sigma_uniform": v["uniform"]["sigma"], "sigma_learned": v["learned"]["sigma"],
+                         "seconds_uniform": v["uniform"]["seconds"], "seconds_learned": v["learned"]["seconds"]}
+                      for k, v in res.items()},
+    }))
+
 
 if __name__ == "__main__":
     main()
