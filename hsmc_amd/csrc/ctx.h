@@ -6,9 +6,10 @@
 //   prep             int32[P]    replica of each particle
 //   cnt              int32[2*C]  occupancy of each sweep cell (two buffers,
 //                                alternating by sweep parity)
-//   spid             int32[C*cap] particle id in each cell slot
-//   sx, sy, sz       FP64[C*cap] position copy in each cell slot
+//   slot             SlotRec[C*cap] position copy and particle id in each
+//                                cell slot (32 B, one cache-line request each)
 //   reps             RepDev[R]   box, grid and RNG key of each replica
+//   swp              SweepDev[R] the current sweep's grid shift and sublattice order
 //   aoff             int32[R+1]  prefix of active cells per phase
 //   arep             int32[A]    replica owning each active cell
 //   sweep            uint64      sweeps done (read by every sweep kernel)
@@ -41,6 +42,24 @@ struct RepDev {
   uint32_t k0, k1;       // Philox key
 };
 
+// What one sweep of a replica draws before any move: the grid shift and the
+// order of the 8 sublattices (nibble p = sublattice of phase p).  k_rebin
+// writes it; the 8 phase launches of the sweep read it.
+struct SweepDev {
+  double shx, shy, shz;
+  uint32_t order;
+  uint32_t pad;
+};
+
+// One cell slot: a sphere's position and its id in its replica.  32 B and
+// 32-B aligned, so one sphere is one cache-line request however many of its
+// fields a kernel reads or writes.
+struct __attribute__((aligned(32))) SlotRec {
+  double x, y, z;
+  int id;
+  int pad;
+};
+
 struct RepHost {
   hsmc_gpu_replica p;
   int gx, gy, gz;
@@ -58,6 +77,7 @@ struct hsmc_gpu_ctx {
   int cap = 8;
   size_t cell_alloc = 0;  // cells allocated (>= Ctot)
   hsmc::RepDev *d_reps = nullptr;
+  hsmc::SweepDev *d_swp = nullptr;  // R entries: the current sweep's shift and order
   int *d_aoff = nullptr;
   int *d_arep = nullptr;  // replica of each active cell (Atot entries)
   size_t arep_alloc = 0;
@@ -65,8 +85,7 @@ struct hsmc_gpu_ctx {
   int *d_prep = nullptr;
   double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr;
   int *d_cnt = nullptr;
-  int *d_spid = nullptr;
-  double *d_sx = nullptr, *d_sy = nullptr, *d_sz = nullptr;
+  hsmc::SlotRec *d_slot = nullptr;
   unsigned long long *d_sweep = nullptr;
   unsigned int *d_cacc = nullptr, *d_cleft = nullptr;
   // host totals per replica: accepted, left-cell (folded from the per-cell

@@ -209,18 +209,13 @@ int layout(hsmc_gpu_ctx *ctx) {
   const size_t need = (size_t)coff;
   if (need > ctx->cell_alloc || cap != ctx->cap || !ctx->d_cnt) {
     (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(ctx->d_cnt); (void)hipFree(ctx->d_spid);
-    (void)hipFree(ctx->d_sx); (void)hipFree(ctx->d_sy); (void)hipFree(ctx->d_sz);
+    (void)hipFree(ctx->d_cnt); (void)hipFree(ctx->d_slot);
     (void)hipFree(ctx->d_cacc); (void)hipFree(ctx->d_cleft);
-    ctx->d_cnt = nullptr; ctx->d_spid = nullptr;
-    ctx->d_sx = ctx->d_sy = ctx->d_sz = nullptr;
+    ctx->d_cnt = nullptr; ctx->d_slot = nullptr;
     ctx->d_cacc = ctx->d_cleft = nullptr;
     size_t alloc = need + need / 8 + 64;
     HSMC_HIP(ctx, hipMalloc(&ctx->d_cnt, sizeof(int) * 2 * alloc));
-    HSMC_HIP(ctx, hipMalloc(&ctx->d_spid, sizeof(int) * alloc * cap));
-    HSMC_HIP(ctx, hipMalloc(&ctx->d_sx, sizeof(double) * alloc * cap));
-    HSMC_HIP(ctx, hipMalloc(&ctx->d_sy, sizeof(double) * alloc * cap));
-    HSMC_HIP(ctx, hipMalloc(&ctx->d_sz, sizeof(double) * alloc * cap));
+    HSMC_HIP(ctx, hipMalloc(&ctx->d_slot, sizeof(SlotRec) * alloc * cap));
     HSMC_HIP(ctx, hipMalloc(&ctx->d_cacc, sizeof(unsigned int) * alloc));
     HSMC_HIP(ctx, hipMalloc(&ctx->d_cleft, sizeof(unsigned int) * alloc));
     ctx->cell_alloc = alloc;
@@ -351,6 +346,7 @@ int hsmc_gpu_create(hsmc_gpu_ctx **out, int device, int n_replicas,
   if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess)
     return fail(HSMC_ERR_HIP);
   if (hipMalloc(&ctx->d_reps, sizeof(RepDev) * n_replicas) != hipSuccess ||
+      hipMalloc(&ctx->d_swp, sizeof(SweepDev) * n_replicas) != hipSuccess ||
       hipMalloc(&ctx->d_aoff, sizeof(int) * (n_replicas + 1)) != hipSuccess ||
       hipMalloc(&ctx->d_prep, sizeof(int) * ptot) != hipSuccess ||
       hipMalloc(&ctx->d_x, sizeof(double) * ptot) != hipSuccess ||
@@ -361,6 +357,7 @@ int hsmc_gpu_create(hsmc_gpu_ctx **out, int device, int n_replicas,
     return fail(HSMC_ERR_NO_MEMORY);
   if (hipMemcpy(ctx->d_prep, prep.data(), sizeof(int) * ptot, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(ctx->d_sweep, 0, sizeof(unsigned long long)) != hipSuccess ||
+      hipMemset(ctx->d_swp, 0, sizeof(SweepDev) * n_replicas) != hipSuccess ||
       hipMemset(ctx->d_err, 0, sizeof(unsigned int)) != hipSuccess ||
       hipMemset(ctx->d_x, 0, sizeof(double) * ptot) != hipSuccess ||
       hipMemset(ctx->d_y, 0, sizeof(double) * ptot) != hipSuccess ||
@@ -376,8 +373,8 @@ int hsmc_gpu_destroy(hsmc_gpu_ctx *ctx) {
   if (!ctx) return HSMC_OK;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   drop_graph(ctx);
-  void *bufs[] = {ctx->d_reps, ctx->d_aoff, ctx->d_arep, ctx->d_prep, ctx->d_x, ctx->d_y, ctx->d_z,
-                  ctx->d_cnt, ctx->d_spid, ctx->d_sx, ctx->d_sy, ctx->d_sz, ctx->d_sweep,
+  void *bufs[] = {ctx->d_reps, ctx->d_swp, ctx->d_aoff, ctx->d_arep, ctx->d_prep, ctx->d_x, ctx->d_y, ctx->d_z,
+                  ctx->d_cnt, ctx->d_slot, ctx->d_sweep,
                   ctx->d_cacc, ctx->d_cleft, ctx->d_err};
   for (void *b : bufs)
     if (b) (void)hipFree(b);

@@ -44,13 +44,12 @@ __host__ __device__ __forceinline__ double u32_unit_closed(uint32_t u) {
   return (double)u / 4294967295.0;
 }
 
-// Minimum image of one component (moves.c:419-424).
+// Minimum image of one component (moves.c:419-424): d - L above half a box,
+// d + L below minus half a box.  Both candidates are computed and one is
+// selected, so the compiler emits selects rather than branches.
 __host__ __device__ __forceinline__ double min_image(double d, double L, double half) {
-  if (d > half)
-    d -= L;
-  else if (d < -half)
-    d += L;
-  return d;
+  const double dm = d - L, dp = d + L;
+  return d > half ? dm : (d < -half ? dp : d);
 }
 
 // Squared distance of the reference's compute_dist with unit scale
@@ -79,13 +78,11 @@ __host__ __device__ __forceinline__ double dist2_scaled(double ax, double ay, do
   return dx * dx + dy * dy + dz * dz;
 }
 
-// apply_pbc for one coordinate (moves.c:215-226): a single wrap.
+// apply_pbc for one coordinate (moves.c:215-226): a single wrap (selected,
+// not branched, as in min_image).
 __host__ __device__ __forceinline__ double wrap_once(double x, double L) {
-  if (x > L)
-    x -= L;
-  else if (x < 0.0)
-    x += L;
-  return x;
+  const double xm = x - L, xp = x + L;
+  return x > L ? xm : (x < 0.0 ? xp : x);
 }
 
 // Cell of coordinate x on a grid of g cells of side `side` whose origin is
@@ -95,11 +92,10 @@ __host__ __device__ __forceinline__ double wrap_once(double x, double L) {
 // minimum side of 1 + 1e-9 diameters.
 __host__ __device__ __forceinline__ int shifted_cell(double x, double shift, double L,
                                                      double inv_side, int g) {
-  double t = x - shift;
-  if (t < 0.0) t += L;
-  int c = (int)(t * inv_side);
-  if (c >= g) c = g - 1;
-  return c;
+  const double t0 = x - shift, t1 = t0 + L;
+  const double t = t0 < 0.0 ? t1 : t0;
+  const int c = (int)(t * inv_side);
+  return c < g ? c : g - 1;
 }
 
 // Cell of coordinate x on an unshifted grid (cell_part_idx, cell_list.c:244-246),

@@ -47,9 +47,7 @@ constexpr uint32_t kNptTag = 0x4E505400u;  // "NPT\0"
 // fl(a - b) = -fl(b - a)).  A thread stops once its replica has an overlap.
 __global__ void __launch_bounds__(256)
 k_vol_overlap(const RepDev *__restrict__ reps, const int *__restrict__ prep, int P, int C,
-              const int *__restrict__ cnt, const int *__restrict__ spid,
-              const double *__restrict__ sx, const double *__restrict__ sy,
-              const double *__restrict__ sz, int cap,
+              const int *__restrict__ cnt, const SlotRec *__restrict__ slot, int cap,
               const unsigned long long *__restrict__ sweep_next, const double *__restrict__ x,
               const double *__restrict__ y, const double *__restrict__ z,
               const double *__restrict__ sfr, int *__restrict__ flags) {
@@ -83,10 +81,10 @@ k_vol_overlap(const RepDev *__restrict__ reps, const int *__restrict__ prep, int
                   wrap_index(cz + dz, rp.gz);
     const int n = min(cntb[c], cap);
     for (int q = 0; q < n; ++q) {
-      const size_t s = (size_t)c * cap + q;
-      if (d == 13 && spid[s] <= me) continue;  // own cell: each pair once, self excluded
+      const SlotRec &rec = slot[(size_t)c * cap + q];
+      if (d == 13 && rec.id <= me) continue;  // own cell: each pair once, self excluded
       // compute_dist with sf (moves.c:400-431), reference particle first
-      if (dist2_scaled(ax, ay, az, sx[s], sy[s], sz[s], rp.lx, rp.ly, rp.lz, sf) < 1.0) {
+      if (dist2_scaled(ax, ay, az, rec.x, rec.y, rec.z, rp.lx, rp.ly, rp.lz, sf) < 1.0) {
         ov = true;
         break;
       }
@@ -182,9 +180,8 @@ int hsmc_gpu_any_overlap_scaled_batch(hsmc_gpu_ctx *ctx, const double *sf, uint8
   hipLaunchKernelGGL(k_zero_flags, dim3((R + 255) / 256), dim3(256), 0, ctx->stream, d_flag, R);
   const unsigned blocks = (unsigned)((ctx->Ptot + 255) / 256);
   hipLaunchKernelGGL(k_vol_overlap, dim3(blocks), dim3(256), 0, ctx->stream, ctx->d_reps,
-                     ctx->d_prep, (int)ctx->Ptot, (int)ctx->Ctot, ctx->d_cnt, ctx->d_spid,
-                     ctx->d_sx, ctx->d_sy, ctx->d_sz, ctx->cap, ctx->d_sweep, ctx->d_x, ctx->d_y,
-                     ctx->d_z, d_sf, d_flag);
+                     ctx->d_prep, (int)ctx->Ptot, (int)ctx->Ctot, ctx->d_cnt, ctx->d_slot,
+                     ctx->cap, ctx->d_sweep, ctx->d_x, ctx->d_y, ctx->d_z, d_sf, d_flag);
   HSMC_HIP(ctx, hipGetLastError());
   std::vector<int> hf(R);
   HSMC_HIP(ctx, hipMemcpyAsync(hf.data(), d_flag, sizeof(int) * R, hipMemcpyDeviceToHost, ctx->stream));

@@ -51,10 +51,10 @@ __device__ __forceinline__ void sweep_shift(const RepDev &rp, unsigned long long
   shz = ((double)u[2] * 0x1p-32) * rp.sz;
 }
 
-// Sublattice visited in phase `phase`: a Fisher-Yates order of 0..7 drawn
-// from one Philox block, kept as 8 nibbles of a register.
-__device__ __forceinline__ int sweep_colour(const RepDev &rp, unsigned long long sweep,
-                                            int phase) {
+// Order of the 8 sublattices in one sweep: a Fisher-Yates order of 0..7
+// drawn from one Philox block, kept as 8 nibbles (nibble p = sublattice of
+// phase p).
+__device__ __forceinline__ uint32_t sweep_order(const RepDev &rp, unsigned long long sweep) {
   uint32_t u[4];
   philox4x32_10(0xFFFFFFFEu, 0u, (uint32_t)sweep, (uint32_t)(sweep >> 32), rp.k0, rp.k1, u);
   uint32_t pk = 0x76543210u;
@@ -67,19 +67,27 @@ __device__ __forceinline__ int sweep_colour(const RepDev &rp, unsigned long long
     pk &= ~((0xFu << (4 * i)) | (0xFu << (4 * j)));
     pk |= (vi << (4 * j)) | (vj << (4 * i));
   }
-  return (int)((pk >> (4 * phase)) & 0xFu);
+  return pk;
 }
 
 __global__ void __launch_bounds__(256)
-k_rebin(const RepDev *__restrict__ reps, const int *__restrict__ prep, int P, int C,
+k_rebin(const RepDev *__restrict__ reps, SweepDev *__restrict__ swp, int R,
+        const int *__restrict__ prep, int P, int C,
         const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
-        int *__restrict__ cnt, int *__restrict__ spid, double *__restrict__ sx,
-        double *__restrict__ sy, double *__restrict__ sz, int cap,
+        int *__restrict__ cnt, SlotRec *__restrict__ slot, int cap,
         const unsigned long long *__restrict__ sweep_base, int k, unsigned *__restrict__ err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const unsigned long long sweep = *sweep_base + (unsigned long long)(long long)k;
   const int buf = (int)(sweep & 1ull);
   if (i < C) cnt[(size_t)(buf ^ 1) * C + i] = 0;  // next sweep's buffer
+  if (i < R) {  // replica i's shift and sublattice order, read by this sweep's phases
+    const RepDev ri = reps[i];
+    SweepDev d;
+    sweep_shift(ri, sweep, d.shx, d.shy, d.shz);
+    d.order = sweep_order(ri, sweep);
+    d.pad = 0u;
+    swp[i] = d;
+  }
   if (i >= P) return;
   const RepDev rp = reps[prep[i]];
   double shx, shy, shz;
@@ -89,16 +97,29 @@ k_rebin(const RepDev *__restrict__ reps, const int *__restrict__ prep, int P, in
   const int cy = shifted_cell(py, shy, rp.ly, rp.iy, rp.gy);
   const int cz = shifted_cell(pz, shz, rp.lz, rp.iz, rp.gz);
   const int c = rp.coff + (cx * rp.gy + cy) * rp.gz + cz;
-  const int slot = atomicAdd(&cnt[(size_t)buf * C + c], 1);
-  if (slot < cap) {
-    const size_t s = (size_t)c * cap + slot;
-    spid[s] = i - rp.poff;
-    sx[s] = px;
-    sy[s] = py;
-    sz[s] = pz;
+  const int sl = atomicAdd(&cnt[(size_t)buf * C + c], 1);
+  if (sl < cap) {
+    SlotRec rec;
+    rec.x = px;
+    rec.y = py;
+    rec.z = pz;
+    rec.id = i - rp.poff;
+    rec.pad = 0;
+    slot[(size_t)c * cap + sl] = rec;
   } else {
     atomicOr(err, kErrOverflow);
   }
+}
+
+// Value of v in lane t of the wave, for a t that is the same in every lane:
+// v_readlane, an SGPR read with no LDS round trip.  Used when a lane group is
+// the whole wave; with two 32-lane groups per wave the halves need different
+// lanes, and ds_bpermute (__shfl) is faster than two readlanes and a select.
+__device__ __forceinline__ double wave_lane(double v, int t) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, t);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), t);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // A lane's copies of its share of the stencil list (entries lane,
@@ -108,11 +129,11 @@ using dvec = double __attribute__((ext_vector_type(M)));
 
 template <int CAP, int LPC, int LCAP>
 __global__ void __launch_bounds__(256)
-k_phase(const RepDev *__restrict__ reps, const int *__restrict__ arep, int R, int A, int phase,
+k_phase(const RepDev *__restrict__ reps, const SweepDev *__restrict__ swp,
+        const int *__restrict__ arep, int R, int A, int phase,
         int k, const unsigned long long *__restrict__ sweep_base, double *__restrict__ x,
         double *__restrict__ y, double *__restrict__ z, const int *__restrict__ cnt, int C,
-        const int *__restrict__ spid, double *__restrict__ sx, double *__restrict__ sy,
-        double *__restrict__ sz, unsigned int *__restrict__ cacc,
+        SlotRec *__restrict__ slot, unsigned int *__restrict__ cacc,
         unsigned int *__restrict__ cleft, unsigned int *__restrict__ err) {
   // The stencil list (the spheres of the 27 neighbour cells, at most LCAP, a
   // bound the host derives from the cell side) lives in registers: entry e
@@ -131,11 +152,11 @@ k_phase(const RepDev *__restrict__ reps, const int *__restrict__ arep, int R, in
   if (a >= A) return;  // the whole lane group leaves together
   const int r = R == 1 ? 0 : arep[a];
   const RepDev rp = reps[r];
+  const SweepDev sw = swp[r];
   const unsigned long long sweep = *sweep_base + (unsigned long long)k;
   const int *cntb = cnt + (size_t)(sweep & 1ull) * C;
-  const int colour = sweep_colour(rp, sweep, phase);
-  double shx, shy, shz;
-  sweep_shift(rp, sweep, shx, shy, shz);
+  const int colour = (int)((sw.order >> (4 * phase)) & 0xFu);
+  const double shx = sw.shx, shy = sw.shy, shz = sw.shz;
 
   // active cell of this work item
   const int hz2 = rp.gz >> 1, hy2 = rp.gy >> 1;
@@ -148,6 +169,13 @@ k_phase(const RepDev *__restrict__ reps, const int *__restrict__ arep, int R, in
   const int cy = 2 * iy2 + ((colour >> 1) & 1);
   const int cz = 2 * iz2 + ((colour >> 2) & 1);
   const int cid = (cx * rp.gy + cy) * rp.gz + cz;
+
+  // draw block of attempt t = lane for every attempt a cell can make (t <
+  // CAP), computed while the counts below are in flight
+  uint32_t w[4] = {0u, 0u, 0u, 0u};
+  if (lane < CAP)
+    philox4x32_10((uint32_t)cid, (uint32_t)lane, (uint32_t)sweep, (uint32_t)(sweep >> 32), rp.k0,
+                  rp.k1, w);
 
   // lane j < 27 owns neighbour cell (j/9-1, (j/3)%3-1, j%3-1); 13 is self
   int nj = 0, ncell = 0;
@@ -182,9 +210,6 @@ k_phase(const RepDev *__restrict__ reps, const int *__restrict__ arep, int R, in
   double my_dx = 0.0, my_dy = 0.0, my_dz = 0.0;
   if (lane < nc) {
     const uint32_t scale = 0xFFFFFFFFu / (uint32_t)nc;
-    uint32_t w[4];
-    philox4x32_10((uint32_t)cid, (uint32_t)lane, (uint32_t)sweep, (uint32_t)(sweep >> 32), rp.k0,
-                  rp.k1, w);
     uint32_t kk = w[0] / scale;
     for (uint32_t rr = 1; kk >= (uint32_t)nc; ++rr) {
       uint32_t w2[4];
@@ -209,13 +234,13 @@ k_phase(const RepDev *__restrict__ reps, const int *__restrict__ arep, int R, in
     const int q = lane + j * LPC;
     px[j] = py[j] = pz[j] = 0.0;
     if (q < T) {
-      const int s = Lm[q];
-      px[j] = sx[s];
-      py[j] = sy[s];
-      pz[j] = sz[s];
+      const SlotRec *rec = slot + Lm[q];
+      px[j] = rec->x;
+      py[j] = rec->y;
+      pz[j] = rec->z;
     }
   }
-  if (lane < nc) Sp[lane] = spid[Lm[oc + lane]];
+  if (lane < nc) Sp[lane] = slot[Lm[oc + lane]].id;
   wave_lds_sync();
   // occupants of the active cell in ascending particle id; attempt t moves
   // list entry oc + Lo[k_t]
@@ -230,23 +255,43 @@ k_phase(const RepDev *__restrict__ reps, const int *__restrict__ arep, int R, in
 
   unsigned int n_acc = 0, n_left = 0;
   for (int t = 0; t < nc; ++t) {
-    const int e = __shfl(my_e, t, LPC);
-    const double ddx = __shfl(my_dx, t, LPC);
-    const double ddy = __shfl(my_dy, t, LPC);
-    const double ddz = __shfl(my_dz, t, LPC);
-    const int owner = e % LPC, je = e / LPC;
-    const double ex = __shfl((double)px[je], owner, LPC);
-    const double ey = __shfl((double)py[je], owner, LPC);
-    const double ez = __shfl((double)pz[je], owner, LPC);
+    // attempt t of each lane group: its draw, then the moving sphere's
+    // current position from the lane holding it
+    unsigned e;
+    double ddx, ddy, ddz, ex, ey, ez;
+    int owner, je;
+    if constexpr (LPC == 64) {
+      e = (unsigned)__builtin_amdgcn_readlane(my_e, t);
+      ddx = wave_lane(my_dx, t);
+      ddy = wave_lane(my_dy, t);
+      ddz = wave_lane(my_dz, t);
+      owner = (int)(e % LPC);
+      je = (int)(e / LPC);
+      ex = wave_lane((double)px[je], owner);
+      ey = wave_lane((double)py[je], owner);
+      ez = wave_lane((double)pz[je], owner);
+    } else {
+      e = (unsigned)__shfl(my_e, t, LPC);
+      ddx = __shfl(my_dx, t, LPC);
+      ddy = __shfl(my_dy, t, LPC);
+      ddz = __shfl(my_dz, t, LPC);
+      owner = (int)(e % LPC);
+      je = (int)(e / LPC);
+      ex = __shfl((double)px[je], owner, LPC);
+      ey = __shfl((double)py[je], owner, LPC);
+      ez = __shfl((double)pz[je], owner, LPC);
+    }
     double nx = ex + ddx;
     double ny = ey + ddy;
     double nz = ez + ddz;
     nx = wrap_once(nx, rp.lx);
     ny = wrap_once(ny, rp.ly);
     nz = wrap_once(nz, rp.lz);
-    if (shifted_cell(nx, shx, rp.lx, rp.ix, rp.gx) != cx ||
-        shifted_cell(ny, shy, rp.ly, rp.iy, rp.gy) != cy ||
-        shifted_cell(nz, shz, rp.lz, rp.iz, rp.gz) != cz) {
+    // all three axes are evaluated (no short circuit), so the test is one branch
+    const bool stays = (shifted_cell(nx, shx, rp.lx, rp.ix, rp.gx) == cx) &
+                       (shifted_cell(ny, shy, rp.ly, rp.iy, rp.gy) == cy) &
+                       (shifted_cell(nz, shz, rp.lz, rp.iz, rp.gz) == cz);
+    if (!stays) {
       ++n_left;
       continue;
     }
@@ -254,9 +299,11 @@ k_phase(const RepDev *__restrict__ reps, const int *__restrict__ arep, int R, in
 #pragma unroll
     for (int j = 0; j < M; ++j) {
       const int q = lane + j * LPC;
-      if (q < T && q != e &&
-          dist2(nx, ny, nz, px[j], py[j], pz[j], rp.lx, rp.ly, rp.lz, rp.hx, rp.hy, rp.hz) < 1.0)
-        ov = true;
+      if (q < T) {
+        const bool hit =
+            dist2(nx, ny, nz, px[j], py[j], pz[j], rp.lx, rp.ly, rp.lz, rp.hx, rp.hy, rp.hz) < 1.0;
+        ov |= hit & (q != (int)e);
+      }
     }
     const unsigned long long bal = __ballot(ov);
     const unsigned long long mine =
@@ -277,10 +324,10 @@ k_phase(const RepDev *__restrict__ reps, const int *__restrict__ arep, int R, in
   for (int j = 0; j < M; ++j) {
     const int q = lane + j * LPC;
     if (q >= oc && q < oc + nc) {
-      const int s = Lm[q];
-      store_through(&sx[s], (double)px[j]);
-      store_through(&sy[s], (double)py[j]);
-      store_through(&sz[s], (double)pz[j]);
+      SlotRec *rec = slot + Lm[q];
+      store_through(&rec->x, (double)px[j]);
+      store_through(&rec->y, (double)py[j]);
+      store_through(&rec->z, (double)pz[j]);
       const int g = rp.poff + Sp[q - oc];
       store_through(&x[g], (double)px[j]);
       store_through(&y[g], (double)py[j]);
@@ -299,10 +346,9 @@ __global__ void k_advance(unsigned long long *sweep_base, int n) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *sweep_base += (unsigned long long)n;
 }
 
-using PhaseFn = void (*)(const RepDev *, const int *, int, int, int, int,
+using PhaseFn = void (*)(const RepDev *, const SweepDev *, const int *, int, int, int, int,
                          const unsigned long long *, double *, double *, double *, const int *,
-                         int, const int *, double *, double *, double *, unsigned int *,
-                         unsigned int *, unsigned int *);
+                         int, SlotRec *, unsigned int *, unsigned int *, unsigned int *);
 
 template <int CAP, int LCAP>
 PhaseFn phase_fn_l(int lpc) {
@@ -332,8 +378,8 @@ bool tuning_ok(const hsmc_gpu_ctx *ctx, const hsmc_gpu_tuning &t) {
 void enqueue_sweep(hsmc_gpu_ctx *ctx, int k, hipStream_t s, hipEvent_t *phase_events) {
   const long work = std::max(ctx->Ptot, ctx->Ctot);
   hipLaunchKernelGGL(k_rebin, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, ctx->d_reps,
-                     ctx->d_prep, (int)ctx->Ptot, (int)ctx->Ctot, ctx->d_x, ctx->d_y, ctx->d_z,
-                     ctx->d_cnt, ctx->d_spid, ctx->d_sx, ctx->d_sy, ctx->d_sz, ctx->cap,
+                     ctx->d_swp, ctx->R, ctx->d_prep, (int)ctx->Ptot, (int)ctx->Ctot, ctx->d_x, ctx->d_y, ctx->d_z,
+                     ctx->d_cnt, ctx->d_slot, ctx->cap,
                      ctx->d_sweep, k, ctx->d_err);
   const int lpc = ctx->tune.lanes_per_cell, cpb = ctx->tune.cells_per_block;
   const unsigned blocks = (unsigned)((ctx->Atot + cpb - 1) / cpb);
@@ -341,10 +387,10 @@ void enqueue_sweep(hsmc_gpu_ctx *ctx, int k, hipStream_t s, hipEvent_t *phase_ev
   PhaseFn fn = phase_fn(ctx->cap, lpc, ctx->lcap);
   for (int p = 0; p < 8; ++p) {
     if (phase_events && p == 0) (void)hipEventRecord(phase_events[0], s);
-    hipLaunchKernelGGL(fn, dim3(blocks), dim3(lpc * cpb), lds, s, ctx->d_reps, ctx->d_arep,
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(lpc * cpb), lds, s, ctx->d_reps, ctx->d_swp, ctx->d_arep,
                        ctx->R, (int)ctx->Atot, p, k, ctx->d_sweep, ctx->d_x, ctx->d_y, ctx->d_z,
-                       ctx->d_cnt, (int)ctx->Ctot, ctx->d_spid, ctx->d_sx, ctx->d_sy, ctx->d_sz,
-                       ctx->d_cacc, ctx->d_cleft, ctx->d_err);
+                       ctx->d_cnt, (int)ctx->Ctot, ctx->d_slot, ctx->d_cacc, ctx->d_cleft,
+                       ctx->d_err);
     if (phase_events && p == 7) (void)hipEventRecord(phase_events[15], s);
   }
 }
@@ -406,8 +452,8 @@ int rebin_last_sweep(hsmc_gpu_ctx *ctx) {
                                ctx->stream));
   const long work = std::max(ctx->Ptot, ctx->Ctot);
   hipLaunchKernelGGL(k_rebin, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, ctx->stream,
-                     ctx->d_reps, ctx->d_prep, (int)ctx->Ptot, (int)ctx->Ctot, ctx->d_x, ctx->d_y,
-                     ctx->d_z, ctx->d_cnt, ctx->d_spid, ctx->d_sx, ctx->d_sy, ctx->d_sz, ctx->cap,
+                     ctx->d_reps, ctx->d_swp, ctx->R, ctx->d_prep, (int)ctx->Ptot, (int)ctx->Ctot, ctx->d_x, ctx->d_y,
+                     ctx->d_z, ctx->d_cnt, ctx->d_slot, ctx->cap,
                      ctx->d_sweep, -1, ctx->d_err);
   HSMC_HIP(ctx, hipGetLastError());
   ctx->cells_version = ctx->version;
