@@ -6,8 +6,11 @@
 //   prep             int32[P]    replica of each particle
 //   cnt              int32[2*C]  occupancy of each sweep cell (two buffers,
 //                                alternating by sweep parity)
-//   slot             SlotRec[C*cap] position copy and particle id in each
-//                                cell slot (32 B, one cache-line request each)
+//   slot             SlotRec[cap*C] position copy and particle id in each
+//                                cell slot (32 B, one cache-line request each),
+//                                slot-major: slot q of cell c is record q*C + c,
+//                                so the first occupants of z-adjacent cells
+//                                share cache lines
 //   reps             RepDev[R]   box, grid and RNG key of each replica
 //   swp              SweepDev[R] the current sweep's grid shift and sublattice order
 //   aoff             int32[R+1]  prefix of active cells per phase
@@ -77,6 +80,7 @@ struct hsmc_gpu_ctx {
   int cap = 8;
   size_t cell_alloc = 0;  // cells allocated (>= Ctot)
   hsmc::RepDev *d_reps = nullptr;
+  hsmc::RepDev rep0{};  // host copy of replica 0's record (a kernel argument when R == 1)
   hsmc::SweepDev *d_swp = nullptr;  // R entries: the current sweep's shift and order
   int *d_aoff = nullptr;
   int *d_arep = nullptr;  // replica of each active cell (Atot entries)

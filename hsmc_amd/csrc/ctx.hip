@@ -120,8 +120,10 @@ bool cells_too_wide(const RepDev &d) { return d.sx >= 1.5 || d.sy >= 1.5 || d.sz
 
 // Box or step-size change that keeps every replica's sweep grid, the slot
 // capacity and the stencil-list bound: only the parameter table changes.
-// The cell arrays, counters and a captured graph stay valid (the kernels
-// read the table at run time; the next sweep re-bins every sphere).
+// The cell arrays and counters stay valid (the next sweep re-bins every
+// sphere), and so does a captured graph of a batch (its kernels read the
+// table at run time); a one-replica graph carries replica 0's record as a
+// kernel argument and is dropped when that record changes.
 bool layout_in_place(hsmc_gpu_ctx *ctx) {
   if (!ctx->d_cnt || !ctx->d_reps) return false;
   std::vector<RepDev> rd(ctx->R);
@@ -140,6 +142,8 @@ bool layout_in_place(hsmc_gpu_ctx *ctx) {
     tmax = std::max(tmax, stencil_list_bound(d));
   }
   if ((big ? 16 : 8) != ctx->cap || lcap_for(tmax) != ctx->lcap) return false;
+  if (ctx->R == 1 && memcmp(&ctx->rep0, &rd[0], sizeof(RepDev)) != 0) drop_graph(ctx);
+  ctx->rep0 = rd[0];
   if (hipMemcpyAsync(ctx->d_reps, rd.data(), sizeof(RepDev) * ctx->R, hipMemcpyHostToDevice,
                      ctx->stream) != hipSuccess ||
       hipStreamSynchronize(ctx->stream) != hipSuccess)  // rd is freed on return
@@ -230,6 +234,7 @@ int layout(hsmc_gpu_ctx *ctx) {
   HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cnt, 0, sizeof(int) * 2 * ctx->cell_alloc, ctx->stream));
   HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cacc, 0, sizeof(unsigned int) * ctx->cell_alloc, ctx->stream));
   HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cleft, 0, sizeof(unsigned int) * ctx->cell_alloc, ctx->stream));
+  ctx->rep0 = rd[0];
   HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_reps, rd.data(), sizeof(RepDev) * ctx->R,
                                hipMemcpyHostToDevice, ctx->stream));
   HSMC_HIP(ctx, hipMemcpyAsync(ctx->d_aoff, ao.data(), sizeof(int) * (ctx->R + 1),

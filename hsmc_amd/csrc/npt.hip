@@ -81,7 +81,7 @@ k_vol_overlap(const RepDev *__restrict__ reps, const int *__restrict__ prep, int
                   wrap_index(cz + dz, rp.gz);
     const int n = min(cntb[c], cap);
     for (int q = 0; q < n; ++q) {
-      const SlotRec &rec = slot[(size_t)c * cap + q];
+      const SlotRec &rec = slot[(size_t)q * C + c];
       if (d == 13 && rec.id <= me) continue;  // own cell: each pair once, self excluded
       // compute_dist with sf (moves.c:400-431), reference particle first
       if (dist2_scaled(ax, ay, az, rec.x, rec.y, rec.z, rp.lx, rp.ly, rp.lz, sf) < 1.0) {

@@ -71,7 +71,7 @@ __device__ __forceinline__ uint32_t sweep_order(const RepDev &rp, unsigned long 
 }
 
 __global__ void __launch_bounds__(256)
-k_rebin(const RepDev *__restrict__ reps, SweepDev *__restrict__ swp, int R,
+k_rebin(const RepDev *__restrict__ reps, const RepDev rp0, SweepDev *__restrict__ swp, int R,
         const int *__restrict__ prep, int P, int C,
         const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
         int *__restrict__ cnt, SlotRec *__restrict__ slot, int cap,
@@ -89,7 +89,7 @@ k_rebin(const RepDev *__restrict__ reps, SweepDev *__restrict__ swp, int R,
     swp[i] = d;
   }
   if (i >= P) return;
-  const RepDev rp = reps[prep[i]];
+  const RepDev rp = R == 1 ? rp0 : reps[prep[i]];
   double shx, shy, shz;
   sweep_shift(rp, sweep, shx, shy, shz);
   const double px = x[i], py = y[i], pz = z[i];
@@ -105,7 +105,7 @@ k_rebin(const RepDev *__restrict__ reps, SweepDev *__restrict__ swp, int R,
     rec.z = pz;
     rec.id = i - rp.poff;
     rec.pad = 0;
-    slot[(size_t)c * cap + sl] = rec;
+    slot[(size_t)sl * C + c] = rec;  // slot-major: slot q of every cell is one array
   } else {
     atomicOr(err, kErrOverflow);
   }
@@ -127,9 +127,11 @@ __device__ __forceinline__ double wave_lane(double v, int t) {
 template <int M>
 using dvec = double __attribute__((ext_vector_type(M)));
 
-template <int CAP, int LPC, int LCAP>
-__global__ void __launch_bounds__(256)
-k_phase(const RepDev *__restrict__ reps, const SweepDev *__restrict__ swp,
+// Occupancy: the 128-entry stencil lists fit 5 waves per SIMD (at most 96
+// VGPRs); the longer lists take 4.
+template <int CAP, int LPC, int LCAP, bool ONE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LCAP <= 128 ? 5 : 4)))
+k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__restrict__ swp,
         const int *__restrict__ arep, int R, int A, int phase,
         int k, const unsigned long long *__restrict__ sweep_base, double *__restrict__ x,
         double *__restrict__ y, double *__restrict__ z, const int *__restrict__ cnt, int C,
@@ -150,11 +152,13 @@ k_phase(const RepDev *__restrict__ reps, const SweepDev *__restrict__ swp,
 
   const int a = blockIdx.x * cpb + sub;
   if (a >= A) return;  // the whole lane group leaves together
-  const int r = R == 1 ? 0 : arep[a];
-  const RepDev rp = reps[r];
-  const SweepDev sw = swp[r];
   const unsigned long long sweep = *sweep_base + (unsigned long long)k;
   const int *cntb = cnt + (size_t)(sweep & 1ull) * C;
+  // the replica's record (a kernel argument when there is one replica) and
+  // the sweep's shift and sublattice order, which the rebin drew
+  const int r = ONE ? 0 : arep[a];
+  const RepDev rp = ONE ? rp0 : reps[r];
+  const SweepDev sw = swp[r];
   const int colour = (int)((sw.order >> (4 * phase)) & 0xFu);
   const double shx = sw.shx, shy = sw.shy, shz = sw.shz;
 
@@ -226,7 +230,7 @@ k_phase(const RepDev *__restrict__ reps, const SweepDev *__restrict__ swp,
   // stage the stencil list: the slot of every entry into LDS, then each
   // lane loads its entries' FP64 positions into registers, all loads of the
   // work item in flight together
-  for (int q = 0; q < nj; ++q) Lm[off + q] = ncell * CAP + q;
+  for (int q = 0; q < nj; ++q) Lm[off + q] = q * C + ncell;
   wave_lds_sync();
   dvec<M> px, py, pz;
 #pragma unroll
@@ -254,6 +258,7 @@ k_phase(const RepDev *__restrict__ reps, const SweepDev *__restrict__ swp,
   const int my_e = lane < nc ? oc + Lo[my_k] : 0;
 
   unsigned int n_acc = 0, n_left = 0;
+  unsigned int moved = 0u;  // bit j: this lane's copy j was moved
   for (int t = 0; t < nc; ++t) {
     // attempt t of each lane group: its draw, then the moving sphere's
     // current position from the lane holding it
@@ -313,17 +318,18 @@ k_phase(const RepDev *__restrict__ reps, const SweepDev *__restrict__ swp,
         px[je] = nx;
         py[je] = ny;
         pz[je] = nz;
+        moved |= 1u << je;
       }
       ++n_acc;
     }
   }
-  // write the active cell's spheres back (slot copy and by-id arrays),
-  // through to the device-coherent level so no dirty L2 lines are left for
-  // the end-of-kernel write-back
+  // write the spheres that moved back (slot copy and by-id arrays), through
+  // to the device-coherent level so no dirty L2 lines are left for the
+  // end-of-kernel write-back
 #pragma unroll
   for (int j = 0; j < M; ++j) {
-    const int q = lane + j * LPC;
-    if (q >= oc && q < oc + nc) {
+    if ((moved >> j) & 1u) {
+      const int q = lane + j * LPC;
       SlotRec *rec = slot + Lm[q];
       store_through(&rec->x, (double)px[j]);
       store_through(&rec->y, (double)py[j]);
@@ -346,23 +352,28 @@ __global__ void k_advance(unsigned long long *sweep_base, int n) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *sweep_base += (unsigned long long)n;
 }
 
-using PhaseFn = void (*)(const RepDev *, const SweepDev *, const int *, int, int, int, int,
+using PhaseFn = void (*)(const RepDev *, const RepDev, const SweepDev *, const int *, int, int, int, int,
                          const unsigned long long *, double *, double *, double *, const int *,
                          int, SlotRec *, unsigned int *, unsigned int *, unsigned int *);
 
-template <int CAP, int LCAP>
+template <int CAP, int LCAP, bool ONE>
 PhaseFn phase_fn_l(int lpc) {
-  return lpc == 64 ? k_phase<CAP, 64, LCAP> : k_phase<CAP, 32, LCAP>;
+  return lpc == 64 ? k_phase<CAP, 64, LCAP, ONE> : k_phase<CAP, 32, LCAP, ONE>;
 }
 
-PhaseFn phase_fn(int cap, int lpc, int lcap) {
+template <bool ONE>
+PhaseFn phase_fn_o(int cap, int lpc, int lcap) {
   if (cap == 8) {
-    if (lcap <= 128) return phase_fn_l<8, 128>(lpc);
-    return lcap <= 192 ? phase_fn_l<8, 192>(lpc) : phase_fn_l<8, 256>(lpc);
+    if (lcap <= 128) return phase_fn_l<8, 128, ONE>(lpc);
+    return lcap <= 192 ? phase_fn_l<8, 192, ONE>(lpc) : phase_fn_l<8, 256, ONE>(lpc);
   }
-  if (lcap <= 128) return phase_fn_l<16, 128>(lpc);
-  if (lcap <= 192) return phase_fn_l<16, 192>(lpc);
-  return phase_fn_l<16, 256>(lpc);
+  if (lcap <= 128) return phase_fn_l<16, 128, ONE>(lpc);
+  if (lcap <= 192) return phase_fn_l<16, 192, ONE>(lpc);
+  return phase_fn_l<16, 256, ONE>(lpc);
+}
+
+PhaseFn phase_fn(int cap, int lpc, int lcap, bool one) {
+  return one ? phase_fn_o<true>(cap, lpc, lcap) : phase_fn_o<false>(cap, lpc, lcap);
 }
 
 
@@ -378,16 +389,17 @@ bool tuning_ok(const hsmc_gpu_ctx *ctx, const hsmc_gpu_tuning &t) {
 void enqueue_sweep(hsmc_gpu_ctx *ctx, int k, hipStream_t s, hipEvent_t *phase_events) {
   const long work = std::max(ctx->Ptot, ctx->Ctot);
   hipLaunchKernelGGL(k_rebin, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, ctx->d_reps,
-                     ctx->d_swp, ctx->R, ctx->d_prep, (int)ctx->Ptot, (int)ctx->Ctot, ctx->d_x, ctx->d_y, ctx->d_z,
+                     ctx->rep0, ctx->d_swp, ctx->R, ctx->d_prep, (int)ctx->Ptot, (int)ctx->Ctot, ctx->d_x, ctx->d_y, ctx->d_z,
                      ctx->d_cnt, ctx->d_slot, ctx->cap,
                      ctx->d_sweep, k, ctx->d_err);
   const int lpc = ctx->tune.lanes_per_cell, cpb = ctx->tune.cells_per_block;
   const unsigned blocks = (unsigned)((ctx->Atot + cpb - 1) / cpb);
   const size_t lds = phase_lds_bytes(ctx->cap, ctx->lcap, cpb);
-  PhaseFn fn = phase_fn(ctx->cap, lpc, ctx->lcap);
+  PhaseFn fn = phase_fn(ctx->cap, lpc, ctx->lcap, ctx->R == 1);
   for (int p = 0; p < 8; ++p) {
     if (phase_events && p == 0) (void)hipEventRecord(phase_events[0], s);
-    hipLaunchKernelGGL(fn, dim3(blocks), dim3(lpc * cpb), lds, s, ctx->d_reps, ctx->d_swp, ctx->d_arep,
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(lpc * cpb), lds, s, ctx->d_reps, ctx->rep0, ctx->d_swp,
+                       ctx->d_arep,
                        ctx->R, (int)ctx->Atot, p, k, ctx->d_sweep, ctx->d_x, ctx->d_y, ctx->d_z,
                        ctx->d_cnt, (int)ctx->Ctot, ctx->d_slot, ctx->d_cacc, ctx->d_cleft,
                        ctx->d_err);
@@ -452,7 +464,7 @@ int rebin_last_sweep(hsmc_gpu_ctx *ctx) {
                                ctx->stream));
   const long work = std::max(ctx->Ptot, ctx->Ctot);
   hipLaunchKernelGGL(k_rebin, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, ctx->stream,
-                     ctx->d_reps, ctx->d_swp, ctx->R, ctx->d_prep, (int)ctx->Ptot, (int)ctx->Ctot, ctx->d_x, ctx->d_y,
+                     ctx->d_reps, ctx->rep0, ctx->d_swp, ctx->R, ctx->d_prep, (int)ctx->Ptot, (int)ctx->Ctot, ctx->d_x, ctx->d_y,
                      ctx->d_z, ctx->d_cnt, ctx->d_slot, ctx->cap,
                      ctx->d_sweep, -1, ctx->d_err);
   HSMC_HIP(ctx, hipGetLastError());
