@@ -203,6 +203,10 @@ class Context:
         self._ck(self.L.hsmc_gpu_set_box(self.h, r, *map(float, box)), "set_box")
         self.replicas[r]["box"] = tuple(box)
 
+    def set_dr_max(self, dr_max: float, r: int = 0) -> None:
+        self._ck(self.L.hsmc_gpu_set_dr_max(self.h, r, float(dr_max)), "set_dr_max")
+        self.replicas[r]["dr_max"] = float(dr_max)
+
     def scale(self, sf: float, box, r: int = 0) -> None:
         self._ck(self.L.hsmc_gpu_scale(self.h, r, float(sf), *map(float, box)), "scale")
         self.replicas[r]["box"] = tuple(box)

@@ -167,6 +167,30 @@ def test_sweeps_graph_and_direct_agree():
     assert np.array_equal(outs[0], outs[1])
 
 
+def test_graph_follows_replica_changes():
+    """A captured sweep graph of a one-replica context carries the replica's
+    record by value: after a step-size change and after a box rescale the
+    graph-replayed run must equal the directly launched one, bit for bit."""
+    o = O.System(2, 8, rho=0.8, neigh_dr=1.0, max_part=10)
+    conf = o.config()
+    L = o.box()["lx"]
+    outs = []
+    for spg in (0, 4):
+        ctx = gpu.Context([dict(n=len(conf), box=(L, L, L), dr_max=0.12, seed=9)])
+        ctx.set_tuning(32, 2, spg)
+        ctx.upload(conf)
+        ctx.sweeps(8)
+        ctx.set_dr_max(0.2)
+        ctx.sweeps(8)
+        sf = 1.0005
+        ctx.scale(sf, (L * sf, L * sf, L * sf))
+        ctx.sweeps(8)
+        outs.append((ctx.download(), ctx.counters()))
+        ctx.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1]
+
+
 def test_full_size_sweeps_keep_hard_core():
     # N = 32000, rho = 0.85 (fcc 4*20^3): 50 sweeps, no overlap, all in box
     conf, box = gpu.lattice(2, 20, 0.85)
