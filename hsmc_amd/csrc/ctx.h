@@ -13,7 +13,8 @@
 //                                share cache lines
 //   reps             RepDev[R]   box, grid and RNG key of each replica
 //   swp              SweepDev[R] the current sweep's grid shift and sublattice order
-//   aoff             int32[R+1]  prefix of active cells per phase
+//   aoff             int32[R+1]  prefix of active cells per phase, each
+//                                replica's range padded to a multiple of 8
 //   arep             int32[A]    replica owning each active cell
 //   sweep            uint64      sweeps done (read by every sweep kernel)
 //   cacc, cleft      uint32[C]   accepted moves / moves rejected for leaving

@@ -57,7 +57,8 @@ void drop_graph(hsmc_gpu_ctx *ctx) {
 
 int fold_counters(hsmc_gpu_ctx *ctx) {
   // the counters are indexed by active-cell index (Atot entries, replica r
-  // owning [aoff, aoff + cells/8))
+  // owning [aoff, aoff + cells/8) and padding up to a multiple of 8 that no
+  // kernel writes)
   if (!ctx->d_cacc || ctx->Atot == 0) return HSMC_OK;
   std::vector<unsigned int> a(ctx->Atot), l(ctx->Atot);
   HSMC_HIP(ctx, hipMemcpyAsync(a.data(), ctx->d_cacc, sizeof(unsigned int) * ctx->Atot,
@@ -176,7 +177,10 @@ int layout(hsmc_gpu_ctx *ctx) {
     h.aoff = aoff;
     const long cells = (long)h.gx * h.gy * h.gz;
     coff += cells;
-    aoff += cells / 8;
+    // a workgroup of the phase kernel takes 1, 2, 4 or 8 consecutive active
+    // cells: padding each replica's range to a multiple of 8 keeps every
+    // workgroup inside one replica (the padding cells exit at once)
+    aoff += (cells / 8 + 7) / 8 * 8;
     RepDev &d = rd[r];
     fill_repdev(h, d);
     if (cells_too_wide(d)) {

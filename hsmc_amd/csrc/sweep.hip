@@ -127,10 +127,10 @@ __device__ __forceinline__ double wave_lane(double v, int t) {
 template <int M>
 using dvec = double __attribute__((ext_vector_type(M)));
 
-// Occupancy: the 128-entry stencil lists fit 5 waves per SIMD (at most 96
-// VGPRs); the longer lists take 4.
+// Occupancy: at least 6 waves per SIMD (at most 80 VGPRs) for stencil lists
+// of up to 192 entries, 4 for the longest lists.
 template <int CAP, int LPC, int LCAP, bool ONE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LCAP <= 128 ? 5 : 4)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LCAP <= 192 ? 6 : 4)))
 k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__restrict__ swp,
         const int *__restrict__ arep, int R, int A, int phase,
         int k, const unsigned long long *__restrict__ sweep_base, double *__restrict__ x,
@@ -155,8 +155,11 @@ k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__res
   const unsigned long long sweep = *sweep_base + (unsigned long long)k;
   const int *cntb = cnt + (size_t)(sweep & 1ull) * C;
   // the replica's record (a kernel argument when there is one replica) and
-  // the sweep's shift and sublattice order, which the rebin drew
-  const int r = ONE ? 0 : arep[a];
+  // the sweep's shift and sublattice order, which the rebin drew.  Each
+  // replica's active cells are padded to a multiple of 8 and cells_per_block
+  // divides 8, so a workgroup lies in one replica: r is uniform and the
+  // record and table are scalar loads.
+  const int r = ONE ? 0 : arep[blockIdx.x * cpb];
   const RepDev rp = ONE ? rp0 : reps[r];
   const SweepDev sw = swp[r];
   const int colour = (int)((sw.order >> (4 * phase)) & 0xFu);
@@ -165,6 +168,7 @@ k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__res
   // active cell of this work item
   const int hz2 = rp.gz >> 1, hy2 = rp.gy >> 1;
   const int la = a - rp.aoff;
+  if (la >= hz2 * hy2 * (rp.gx >> 1)) return;  // padding of the replica's range
   const int iz2 = la % hz2;
   const int t2 = la / hz2;
   const int iy2 = t2 % hy2;
@@ -379,7 +383,11 @@ PhaseFn phase_fn(int cap, int lpc, int lcap, bool one) {
 
 bool tuning_ok(const hsmc_gpu_ctx *ctx, const hsmc_gpu_tuning &t) {
   if (t.lanes_per_cell != 32 && t.lanes_per_cell != 64) return false;
-  if (t.cells_per_block < 1 || t.lanes_per_cell * t.cells_per_block > 256) return false;
+  // 1, 2, 4 or 8 cells per workgroup: a divisor of the padding of each
+  // replica's active-cell range
+  if ((t.cells_per_block & (t.cells_per_block - 1)) != 0 || t.cells_per_block < 1 ||
+      t.cells_per_block > 8 || t.lanes_per_cell * t.cells_per_block > 256)
+    return false;
   if (phase_lds_bytes(ctx->cap, ctx->lcap, t.cells_per_block) > 64 * 1024) return false;
   if (t.sweeps_per_graph < 0 || t.sweeps_per_graph > 512) return false;
   return true;
