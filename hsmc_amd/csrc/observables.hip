@@ -42,28 +42,42 @@ __global__ void k_grid_count(const double *__restrict__ x, const double *__restr
   atomicAdd(&cnt[c], 1);
 }
 
-// Exclusive prefix sum of cnt[0..n) into start[0..n], one workgroup.
+// Exclusive prefix sum of cnt[0..n) into start[0..n], one workgroup.  Each
+// pass covers 32 * 1024 entries: a thread loads its 32 consecutive entries
+// at once (all loads in flight together), the block scans the thread sums,
+// and a carry links the passes.
 __global__ void __launch_bounds__(1024) k_scan1(const int *__restrict__ cnt, int n, int *__restrict__ start) {
+  constexpr int K = 32;
   __shared__ int part[1024];
   const int t = threadIdx.x;
-  const int per = (n + 1023) / 1024;
-  const int b = t * per, e = min(n, b + per);
-  int s = 0;
-  for (int i = b; i < e; ++i) s += cnt[i];
-  part[t] = s;
-  __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
-    int v = (t >= d) ? part[t - d] : 0;
+  int carry = 0;
+  for (int base = 0; base < n; base += K * 1024) {
+    const int b = base + t * K;
+    int v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = (b + k < n) ? cnt[b + k] : 0;
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) s += v[k];
+    part[t] = s;
     __syncthreads();
-    part[t] += v;
-    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+      const int u = (t >= d) ? part[t - d] : 0;
+      __syncthreads();
+      part[t] += u;
+      __syncthreads();
+    }
+    int run = carry + part[t] - s;
+    const int total = part[1023];
+    __syncthreads();  // part is rewritten by the next pass
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (b + k < n) start[b + k] = run;
+      run += v[k];
+    }
+    carry += total;
   }
-  int run = part[t] - s;
-  for (int i = b; i < e; ++i) {
-    start[i] = run;
-    run += cnt[i];
-  }
-  if (t == 1023) start[n] = part[1023];
+  if (t == 0) start[n] = carry;
 }
 
 __global__ void k_grid_scatter(const double *__restrict__ x, const double *__restrict__ y,
@@ -279,6 +293,103 @@ k_pair_hist(Grid g, int n, double dr_bin, int nbins, double rmax_eff, double cut
     if (hist[b]) atomicAdd(&counts[b], (unsigned long long)hist[b]);
   const unsigned long long s = block_sum_u64(bad);
   if (threadIdx.x == 0 && s) atomicAdd(oor, s);
+}
+
+// The same pair histogram on cell pairs, for grids whose cells hold many
+// spheres (the g(r) cutoff of several diameters), where one thread per
+// sphere leaves most of the GPU without waves.  One wave per (cell A,
+// stencil cell B) with B at or after A in cell order: B's sorted positions
+// are staged in LDS a tile at a time and each lane tests its sphere of A
+// against the tile.  Positions are sorted by cell, so "q after p in sorted
+// order" is the per-sphere kernel's rule and every pair is counted once,
+// with the same arithmetic (reference sphere first).
+constexpr int kPairTile = 256;
+
+size_t pair_tiled_lds(int nbins) {
+  return ((sizeof(unsigned int) * (size_t)nbins + 7) & ~(size_t)7) + 3 * sizeof(double) * kPairTile;
+}
+
+// One lane's sphere (px, py, pz) against tile entries [j0, nq): the per-pair
+// arithmetic of k_pair_hist; MI = false leaves out the minimum image where
+// it cannot change a difference.
+template <bool MI>
+__device__ __forceinline__ void pair_tile(const Grid &g, double px, double py, double pz,
+                                          const double *tx, const double *ty, const double *tz,
+                                          int j0, int nq, double cut2, double rmax_eff,
+                                          double dr_bin, int nbins, unsigned int *hist,
+                                          unsigned long long &bad) {
+  for (int j = j0; j < nq; ++j) {
+    double s;
+    if constexpr (MI) {
+      s = dist2(px, py, pz, tx[j], ty[j], tz[j], g.lx, g.ly, g.lz, g.hx, g.hy, g.hz);
+    } else {
+      const double dx = px - tx[j], dy = py - ty[j], dz = pz - tz[j];
+      s = dx * dx + dy * dy + dz * dz;
+    }
+    if (s >= cut2) continue;
+    const double dr = sqrt(s);
+    if (dr < rmax_eff) {
+      const int bin = (int)((dr - 1.0) / dr_bin);
+      if (bin >= 0 && bin < nbins)
+        atomicAdd(&hist[bin], 1u);
+      else
+        ++bad;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64)
+k_pair_hist_tiled(Grid g, int nbins, double dr_bin, double rmax_eff, double cut2,
+                  unsigned long long *__restrict__ counts, unsigned long long *__restrict__ oor) {
+  extern __shared__ unsigned long long smem_u64[];
+  unsigned int *hist = (unsigned int *)smem_u64;
+  double *tx = (double *)((unsigned char *)smem_u64 + ((sizeof(unsigned int) * (size_t)nbins + 7) & ~(size_t)7));
+  double *ty = tx + kPairTile, *tz = ty + kPairTile;
+  const int lane = threadIdx.x;
+  const int cA = (int)(blockIdx.x / 27u), st = (int)(blockIdx.x % 27u);
+  const int dx = st / 9 - 1, dy = (st / 3) % 3 - 1, dz = st % 3 - 1;
+  if (!stencil_axis_ok(dx, g.nx) || !stencil_axis_ok(dy, g.ny) || !stencil_axis_ok(dz, g.nz)) return;
+  const int ax = cA / (g.ny * g.nz), ay = (cA / g.nz) % g.ny, az = cA % g.nz;
+  const int cB = (wrap_index(ax + dx, g.nx) * g.ny + wrap_index(ay + dy, g.ny)) * g.nz +
+                 wrap_index(az + dz, g.nz);
+  if (cB < cA) return;  // counted from B's side
+  const int bA = g.start[cA], eA = g.start[cA + 1], bB = g.start[cB], eB = g.start[cB + 1];
+  if (bA == eA || bB == eB) return;
+  // No wrap between A and B on any axis, and at least 5 cells per axis: every
+  // coordinate difference is within 2 sides (plus rounding of the cell
+  // index) <= 0.4 L, below half a box, where the minimum image returns it
+  // unchanged, so the tile skips it (the same doubles either way).
+  const bool direct = g.nx >= 5 && g.ny >= 5 && g.nz >= 5 && ax + dx >= 0 && ax + dx < g.nx &&
+                      ay + dy >= 0 && ay + dy < g.ny && az + dz >= 0 && az + dz < g.nz;
+  for (int b = lane; b < nbins; b += 64) hist[b] = 0;
+  unsigned long long bad = 0;
+  for (int p0 = bA; p0 < eA; p0 += 64) {
+    const int p = p0 + lane;
+    const bool act = p < eA;
+    const double px = act ? g.px[p] : 0.0, py = act ? g.py[p] : 0.0, pz = act ? g.pz[p] : 0.0;
+    for (int q0 = bB; q0 < eB; q0 += kPairTile) {
+      const int nq = min(kPairTile, eB - q0);
+      __syncthreads();  // the previous tile is no longer read
+      for (int j = lane; j < nq; j += 64) {
+        tx[j] = g.px[q0 + j];
+        ty[j] = g.py[q0 + j];
+        tz[j] = g.pz[q0 + j];
+      }
+      __syncthreads();
+      if (!act) continue;
+      // in A's own cell only the spheres after p; in a later cell all
+      const int j0 = max(0, p + 1 - q0);
+      if (direct)
+        pair_tile<false>(g, px, py, pz, tx, ty, tz, j0, nq, cut2, rmax_eff, dr_bin, nbins, hist, bad);
+      else
+        pair_tile<true>(g, px, py, pz, tx, ty, tz, j0, nq, cut2, rmax_eff, dr_bin, nbins, hist, bad);
+    }
+  }
+  __syncthreads();
+  for (int b = lane; b < nbins; b += 64)
+    if (hist[b]) atomicAdd(&counts[b], (unsigned long long)hist[b]);
+  for (int o = 32; o > 0; o >>= 1) bad += __shfl_down(bad, o, 64);
+  if (lane == 0 && bad) atomicAdd(oor, bad);
 }
 
 int grid_cells_for(double L, double cut) {
@@ -518,8 +629,15 @@ static int pair_hist(hsmc_gpu_ctx *ctx, int replica, double dr_bin, int nbins, d
   if (!dcounts) return HSMC_ERR_NO_MEMORY;
   HSMC_HIP(ctx, hipMemsetAsync(dcounts, 0, sizeof(unsigned long long) * (nbins + 1), ctx->stream));
   const double cut2 = (rmax_eff * (1.0 + 1e-6)) * (rmax_eff * (1.0 + 1e-6));
-  hipLaunchKernelGGL(k_pair_hist, dim3((N + 255) / 256), dim3(256), sizeof(unsigned int) * nbins,
-                     ctx->stream, g, N, dr_bin, nbins, rmax_eff, cut2, dcounts, dcounts + nbins);
+  const long C = (long)g.nx * g.ny * g.nz;
+  if (N >= 16 * C && pair_tiled_lds(nbins) <= 64 * 1024 && 27 * C < (1L << 31)) {
+    // cells of 16 spheres or more on average: one wave per cell pair
+    hipLaunchKernelGGL(k_pair_hist_tiled, dim3((unsigned)(27 * C)), dim3(64), pair_tiled_lds(nbins),
+                       ctx->stream, g, nbins, dr_bin, rmax_eff, cut2, dcounts, dcounts + nbins);
+  } else {
+    hipLaunchKernelGGL(k_pair_hist, dim3((N + 255) / 256), dim3(256), sizeof(unsigned int) * nbins,
+                       ctx->stream, g, N, dr_bin, nbins, rmax_eff, cut2, dcounts, dcounts + nbins);
+  }
   HSMC_HIP(ctx, hipGetLastError());
   std::vector<unsigned long long> hc(nbins + 1);
   HSMC_HIP(ctx, hipMemcpyAsync(hc.data(), dcounts, sizeof(unsigned long long) * (nbins + 1),
