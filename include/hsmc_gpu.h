@@ -57,8 +57,9 @@ typedef struct hsmc_gpu_replica {
 
 /* Launch shape of the sweep kernels (chosen by hsmc_gpu_autotune or set by
  * hand).  lanes_per_cell: 32 or 64 lanes test one active cell;
- * cells_per_block: active cells per workgroup; sweeps_per_graph: sweeps
- * captured in one hipGraph launch (0 = direct launches). */
+ * cells_per_block: active cells per workgroup, 1, 2, 4 or 8 (at most 256
+ * lanes); sweeps_per_graph: sweeps captured in one hipGraph launch (0 =
+ * direct launches, at most 512).  Other values give HSMC_ERR_BAD_ARG. */
 typedef struct hsmc_gpu_tuning {
   int32_t lanes_per_cell;
   int32_t cells_per_block;

@@ -61,6 +61,8 @@ def test_arguments_out_of_range(sys500):
     assert _code(lambda: ctx.rdf_counts(0.01, 400, L / 2 + 0.5)) == BAD_ARG   # beyond minimum image
     assert _code(lambda: ctx.any_overlap_scaled(np.full(65, 0.99))) == BAD_ARG   # at most 64 factors
     assert _code(lambda: ctx.set_tuning(48, 2, 0)) == BAD_ARG
+    assert _code(lambda: ctx.set_tuning(32, 3, 0)) == BAD_ARG   # cells per workgroup: 1, 2, 4 or 8
+    assert _code(lambda: ctx.set_tuning(64, 8, 0)) == BAD_ARG   # at most 256 lanes
     assert _code(lambda: ctx.npt_sweeps(1, -1.0, 0.01)) == BAD_ARG
     assert _code(lambda: ctx.npt_sweeps(1, 3.0, 0.0)) == BAD_ARG
     assert _code(lambda: ctx.overlap_batch(np.array([len(conf)], np.int32), np.zeros((1, 3)))) == BAD_ARG
