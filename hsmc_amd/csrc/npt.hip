@@ -36,35 +36,37 @@ using namespace hsmc;
 
 namespace {
 
-constexpr int S_SF = 40, S_BOX = 41, S_FLAG = 42;
+constexpr int S_SF = 40, S_BOX = 41, S_FLAG = 42, S_BLK = 43;
 constexpr uint32_t kNptTag = 0x4E505400u;  // "NPT\0"
 
 // Any pair of a replica with a proposed compression (sf[r] > 0) closer than 1
 // after scaling: thread per sphere, neighbours from the last sweep's cells.
+// Blocks cover only the replicas that propose a compression: block b takes
+// the 256 spheres from blk[b] on, all of one replica, so the replica (and
+// its record) is uniform and the block stops as a whole once the replica
+// has an overlap.
 // Each pair is tested once: from the sphere's own cell towards the 13
 // lexicographically forward neighbour cells, and within the cell towards
 // higher ids (the scaled distance is the same from either end, as
 // fl(a - b) = -fl(b - a)).  A thread stops once its replica has an overlap.
 __global__ void __launch_bounds__(256)
-k_vol_overlap(const RepDev *__restrict__ reps, const int *__restrict__ prep, int P, int C,
+k_vol_overlap(const RepDev *__restrict__ reps, const SweepDev *__restrict__ swp,
+              const int *__restrict__ prep, const int *__restrict__ blk, int C,
               const int *__restrict__ cnt, const SlotRec *__restrict__ slot, int cap,
               const unsigned long long *__restrict__ sweep_next, const double *__restrict__ x,
               const double *__restrict__ y, const double *__restrict__ z,
               const double *__restrict__ sfr, int *__restrict__ flags) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P) return;
-  const int r = prep[i];
-  const double sf = sfr[r];
-  if (!(sf > 0.0)) return;
-  if (__hip_atomic_load(&flags[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const int i0 = blk[blockIdx.x];
+  const int r = prep[i0];
   const RepDev rp = reps[r];
+  const int i = i0 + (int)threadIdx.x;
+  if (i >= rp.poff + rp.N) return;
+  const double sf = sfr[r];
+  if (__hip_atomic_load(&flags[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const unsigned long long sweep = *sweep_next - 1ull;
   const int *cntb = cnt + (size_t)(sweep & 1ull) * C;
-  uint32_t u[4];
-  philox4x32_10(0xFFFFFFFFu, 0u, (uint32_t)sweep, (uint32_t)(sweep >> 32), rp.k0, rp.k1, u);
-  const double shx = ((double)u[0] * 0x1p-32) * rp.sx;
-  const double shy = ((double)u[1] * 0x1p-32) * rp.sy;
-  const double shz = ((double)u[2] * 0x1p-32) * rp.sz;
+  // the last sweep's grid shift, as its rebin drew it
+  const double shx = swp[r].shx, shy = swp[r].shy, shz = swp[r].shz;
   const double ax = x[i], ay = y[i], az = z[i];
   const int cx = shifted_cell(ax, shx, rp.lx, rp.ix, rp.gx);
   const int cy = shifted_cell(ay, shy, rp.ly, rp.iy, rp.gy);
@@ -176,11 +178,20 @@ int hsmc_gpu_any_overlap_scaled_batch(hsmc_gpu_ctx *ctx, const double *sf, uint8
     int st = rebin_last_sweep(ctx);
     if (st) return st;
   }
+  // first sphere of each 256-sphere block of the replicas scanned here
+  std::vector<int> blk;
+  for (int r = 0; r < R; ++r) {
+    if (!(sf_dev[r] > 0.0)) continue;
+    const RepHost &h = ctx->reps[r];
+    for (long b = h.poff; b < h.poff + h.p.n_part; b += 256) blk.push_back((int)b);
+  }
+  int *d_blk = (int *)scratch(ctx, S_BLK, sizeof(int) * blk.size());
+  if (!d_blk) return HSMC_ERR_NO_MEMORY;
   HSMC_HIP(ctx, hipMemcpyAsync(d_sf, sf_dev.data(), sizeof(double) * R, hipMemcpyHostToDevice, ctx->stream));
+  HSMC_HIP(ctx, hipMemcpyAsync(d_blk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice, ctx->stream));
   hipLaunchKernelGGL(k_zero_flags, dim3((R + 255) / 256), dim3(256), 0, ctx->stream, d_flag, R);
-  const unsigned blocks = (unsigned)((ctx->Ptot + 255) / 256);
-  hipLaunchKernelGGL(k_vol_overlap, dim3(blocks), dim3(256), 0, ctx->stream, ctx->d_reps,
-                     ctx->d_prep, (int)ctx->Ptot, (int)ctx->Ctot, ctx->d_cnt, ctx->d_slot,
+  hipLaunchKernelGGL(k_vol_overlap, dim3((unsigned)blk.size()), dim3(256), 0, ctx->stream, ctx->d_reps,
+                     ctx->d_swp, ctx->d_prep, d_blk, (int)ctx->Ctot, ctx->d_cnt, ctx->d_slot,
                      ctx->cap, ctx->d_sweep, ctx->d_x, ctx->d_y, ctx->d_z, d_sf, d_flag);
   HSMC_HIP(ctx, hipGetLastError());
   std::vector<int> hf(R);
