@@ -120,3 +120,50 @@ def test_npt_batched_density_matches_serial():
     # Carnahan-Starling density at beta P = 3 is 0.6348 (N = 256 sits a little above)
     assert abs(m_gpu - 0.6348) < 0.012
     ctx.close()
+
+
+# The reference's published NpT table (README.md:151-158): N = 256, packing
+# fraction at each input pressure, against Brumby et al. (their values and
+# standard deviations)
+PUBLISHED_NPT = [(1.472, 0.249, 0.003), (2.280, 0.300, 0.002), (3.471, 0.349, 0.002),
+                 (5.274, 0.399, 0.002)]
+
+
+def test_npt_packing_fraction_matches_published_table():
+    """8 replicas of N = 256 per pressure, all 32 in one batched context:
+    the mean packing fraction agrees with the published values within three
+    combined standard errors plus 0.002 for the finite size (N = 256)."""
+    from hsmc_amd import analysis as A
+    K, eq, stat, every = 8, 3000, 15000, 25
+
+    def cs_density(p):
+        lo, hi = 0.1, 0.99
+        for _ in range(100):
+            mid = 0.5 * (lo + hi)
+            lo, hi = (mid, hi) if mid * A.carnahan_starling(mid)["Z"] < p else (lo, mid)
+        return 0.5 * (lo + hi)
+
+    press = [p for p, _, _ in PUBLISHED_NPT for _ in range(K)]
+    dv = [{1.472: 0.03, 2.280: 0.025, 3.471: 0.02, 5.274: 0.015}[p] for p in press]
+    specs = [(4, cs_density(p), 300 + r, 0.15) for r, p in enumerate(press)]
+    ctx = make(specs)
+    ctx.npt_sweeps(eq, press, dv)
+    R = len(press)
+    eta = np.zeros((stat // every, R))
+    moves = np.zeros((R, 4), dtype=np.uint64)
+    for k in range(stat // every):
+        moves += ctx.npt_sweeps(every, press, dv)
+        for r in range(R):
+            b = ctx.box(r)
+            eta[k, r] = math.pi / 6.0 * 256 / (b[0] * b[1] * b[2])
+    assert all(ctx.any_overlap_scaled_batch(1.0) == 1)
+    acc = moves[:, 1].sum() / moves[:, 0].sum()
+    assert 0.03 < acc < 0.9, acc   # about 0.1 with these steps
+    means = eta.mean(axis=0)
+    for i, (p, ref, ref_sd) in enumerate(PUBLISHED_NPT):
+        m = means[i * K:(i + 1) * K]
+        se = max(np.std(m, ddof=1), 0.002) / math.sqrt(K)
+        tol = 3.0 * math.hypot(se, ref_sd) + 0.002
+        assert abs(m.mean() - ref) < tol, (p, m.mean(), ref, se)
+    ctx.close()
+
