@@ -1,8 +1,11 @@
 """Batched replicas: many independent systems of uneven size and density in
 one context sweep together; each must evolve exactly as it would alone."""
+import math
+
 import numpy as np
 import pytest
 
+from hsmc_amd import analysis as A
 from hsmc_amd import gpu
 from oracle import oracle as O
 
@@ -80,3 +83,58 @@ def test_autotune_keeps_state():
     after = [ctx.download(r) for r in range(len(reps))]
     assert all(np.array_equal(a, b) for a, b in zip(before, after))
     assert ctx.counters(0) == c0
+
+
+# The reference's published NVT table (README.md:127-133): N = 256, the
+# virial pressure beta P sigma^3 from the contact value of g, against
+# DeMiguel and Jackson's values (with their standard deviations)
+PUBLISHED_NVT = [(0.25, 1.4688, 0.0009), (0.30, 2.2784, 0.0013), (0.35, 3.4841, 0.0019),
+                 (0.40, 5.288, 0.003), (0.45, 8.030, 0.004)]
+
+
+def virial_pressure(cnt, rho, n):
+    """beta P sigma^3 from one contact histogram (unordered pair counts in 5
+    bins of 0.01 from r = 1), the reference's route (compute_press.c:123-187,
+    hsmc_pressure.py:11-41): g per shell, a straight-line fit extrapolated
+    to contact, Z = 1 + (2 pi / 3) rho g(1+)."""
+    rr = np.array([1.0 + (i + 0.5) * 0.01 for i in range(5)])
+    shells = 4.0 * math.pi / 3.0 * ((rr + 0.005) ** 3 - (rr - 0.005) ** 3)
+    g = 2.0 * np.asarray(cnt, dtype=float) / (shells * rho * n)
+    b = A.linear_fit(rr, g)
+    return rho * (1.0 + 2.0 * math.pi / 3.0 * rho * (b[0] + b[1]))
+
+
+def test_nvt_pressure_matches_published_table():
+    """8 replicas of N = 256 per packing fraction, all 40 in one batched
+    context: the mean virial pressure agrees with the published values
+    within three combined standard errors plus 1% (the straight-line
+    extrapolation to contact over 0.05 diameters)."""
+    K, eq, samples, every = 8, 10000, 300, 10
+    reps, confs = [], []
+    for i, (eta, _, _) in enumerate(PUBLISHED_NVT):
+        conf, box = gpu.lattice(2, 4, 6.0 * eta / math.pi)
+        for k in range(K):
+            reps.append(dict(n=len(conf), box=box, dr_max=0.1, seed=900 + 10 * i + k))
+            confs.append(conf)
+    ctx = gpu.Context(reps)
+    for r, c in enumerate(confs):
+        ctx.upload(c, r)
+    ctx.sweeps(eq)
+    R = len(reps)
+    press = np.zeros((samples, R))
+    for s in range(samples):
+        for r in range(R):
+            cnt, oor = ctx.contact_counts(0.01, 5, 1.05, r)
+            assert oor == 0
+            rho = 256 / float(np.prod(reps[r]["box"]))
+            press[s, r] = virial_pressure(cnt, rho, 256)
+        ctx.sweeps(every)
+    means = press.mean(axis=0)
+    got = []
+    for i, (eta, ref, ref_sd) in enumerate(PUBLISHED_NVT):
+        m = means[i * K:(i + 1) * K]
+        se = np.std(m, ddof=1) / math.sqrt(K)
+        got.append((eta, round(m.mean(), 4), round(se, 4), ref))
+        assert abs(m.mean() - ref) < 3.0 * math.hypot(se, ref_sd) + 0.01 * ref, got
+    ctx.close()
+
