@@ -138,3 +138,43 @@ def test_nvt_pressure_matches_published_table():
         assert abs(m.mean() - ref) < 3.0 * math.hypot(se, ref_sd) + 0.01 * ref, got
     ctx.close()
 
+
+# The reference's published Widom table (README.md:170-177): N = 256, beta
+# mu_ex against Adams.  The reference's own value at rho = 0.7 (6.656) missed
+# by 10% for want of successful insertions; 10^5 insertions per sample on 8
+# replicas reach it.
+PUBLISHED_MU = [(0.1, 0.466), (0.3, 1.77), (0.5, 3.832), (0.7, 7.391)]
+
+
+def test_widom_mu_matches_published_table():
+    """8 replicas of N = 256 per density in one batched context, 300 samples
+    of 10^5 uniform insertions: beta mu_ex within three standard errors plus
+    0.03 of Adams' values (their own uncertainty and the finite size)."""
+    K, eq, samples, every, ins = 8, 10000, 300, 10, 100000
+    reps, confs = [], []
+    for i, (rho, _) in enumerate(PUBLISHED_MU):
+        conf, box = gpu.lattice(2, 4, rho)
+        for k in range(K):
+            reps.append(dict(n=len(conf), box=box, dr_max=0.1 if rho > 0.4 else 0.3,
+                             seed=1300 + 10 * i + k))
+            confs.append(conf)
+    ctx = gpu.Context(reps)
+    for r, c in enumerate(confs):
+        ctx.upload(c, r)
+    ctx.sweeps(eq)
+    R = len(reps)
+    hits = np.zeros((samples, R))
+    for s in range(samples):
+        for r in range(R):
+            hits[s, r] = ctx.widom_uniform(77 + r, s, 0, ins, r)
+        ctx.sweeps(every)
+    p = hits.mean(axis=0) / ins
+    got = []
+    for i, (rho, ref) in enumerate(PUBLISHED_MU):
+        pr = p[i * K:(i + 1) * K]
+        mu = -math.log(pr.mean())
+        se = np.std(pr, ddof=1) / math.sqrt(K) / pr.mean()
+        got.append((rho, round(mu, 4), round(se, 4), ref))
+        assert abs(mu - ref) < 3.0 * se + 0.03, got
+    ctx.close()
+
