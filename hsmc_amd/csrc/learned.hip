@@ -586,7 +586,7 @@ k_cdf_local(const float *__restrict__ f, long V, double fmax, double sumexp, dou
   s[threadIdx.x] = tot;
   __syncthreads();
   for (int d = 1; d < 1024; d <<= 1) {
-    const double t = threadIdx.x >= d ? s[threadIdx.x - d] : 0.0;
+    const double t = (int)threadIdx.x >= d ? s[threadIdx.x - d] : 0.0;
     __syncthreads();
     s[threadIdx.x] += t;
     __syncthreads();

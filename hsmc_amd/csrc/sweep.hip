@@ -297,9 +297,9 @@ k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__res
     ny = wrap_once(ny, rp.ly);
     nz = wrap_once(nz, rp.lz);
     // all three axes are evaluated (no short circuit), so the test is one branch
-    const bool stays = (shifted_cell(nx, shx, rp.lx, rp.ix, rp.gx) == cx) &
-                       (shifted_cell(ny, shy, rp.ly, rp.iy, rp.gy) == cy) &
-                       (shifted_cell(nz, shz, rp.lz, rp.iz, rp.gz) == cz);
+    const bool stays = ((int)(shifted_cell(nx, shx, rp.lx, rp.ix, rp.gx) == cx) &
+                        (int)(shifted_cell(ny, shy, rp.ly, rp.iy, rp.gy) == cy) &
+                        (int)(shifted_cell(nz, shz, rp.lz, rp.iz, rp.gz) == cz)) != 0;
     if (!stays) {
       ++n_left;
       continue;
