@@ -290,6 +290,18 @@ static void set_gpu_dr_max(void) {
   if (g) host_gpu_check(hsmc_gpu_set_dr_max(g, 0, G_IN.dr_max), "set_dr_max");
 }
 
+/* One secant step of the optimizer (optimizer.c:47, :117-118) towards the
+ * target acceptance.  When two samples give the same acceptance the
+ * reference divides by zero and carries an infinite or NaN step into the
+ * next moves (an infinite volume step is then accepted through exp(NaN));
+ * here the step keeps its last value instead. */
+static double secant_step(double x2, double x1, double a2, double a1, double target) {
+  const double den = a2 - a1;
+  if (den == 0.0) return x2;
+  const double x = x2 - (a2 - target) * (x2 - x1) / den;
+  return isfinite(x) ? x : x2;
+}
+
 void opt_nvt(void) {
   int sample_iter = G_IN.opt_sweeps / G_IN.opt_samples;
   double acc_ratio_1, acc_ratio_2, dr_1, dr_2;
@@ -303,7 +315,7 @@ void opt_nvt(void) {
   set_gpu_dr_max();
   get_sample_nvt(&dr_2, &acc_ratio_2, sample_iter);
   for (int ii = 0; ii < G_IN.opt_samples; ii++) {
-    G_IN.dr_max = dr_2 - (acc_ratio_2 - G_IN.opt_part_target) * (dr_2 - dr_1) / (acc_ratio_2 - acc_ratio_1);
+    G_IN.dr_max = secant_step(dr_2, dr_1, acc_ratio_2, acc_ratio_1, G_IN.opt_part_target);
     if (G_IN.dr_max > 1.0) {
       G_IN.dr_max = 1.0;
     } else if (G_IN.dr_max <= 0.0) {
@@ -347,8 +359,8 @@ void opt_npt(void) {
   set_gpu_dr_max();
   get_sample_npt(&dr_2, &ap2, &dv_2, &av2, sample_iter);
   for (int ii = 0; ii < G_IN.opt_samples; ii++) {
-    G_IN.dr_max = dr_2 - (ap2 - G_IN.opt_part_target) * (dr_2 - dr_1) / (ap2 - ap1);
-    G_IN.dv_max = dv_2 - (av2 - G_IN.opt_vol_target) * (dv_2 - dv_1) / (av2 - av1);
+    G_IN.dr_max = secant_step(dr_2, dr_1, ap2, ap1, G_IN.opt_part_target);
+    G_IN.dv_max = secant_step(dv_2, dv_1, av2, av1, G_IN.opt_vol_target);
     if (G_IN.dr_max > 1.0) G_IN.dr_max = 1.0;
     if (G_IN.dr_max <= 0.0) {
       G_IN.dr_max = -G_IN.dr_max;

@@ -280,6 +280,35 @@ out 1000
     assert abs(final - 0.5) < 0.12
 
 
+def test_npt_optimizer_survives_equal_acceptances(tmp_path):
+    """With about 20 volume moves per optimizer sample, two samples can give
+    the same volume acceptance (here both 0 at the tenth sample), where the
+    reference's secant step (optimizer.c:117-118) divides by zero.  The step
+    keeps its last value and the run completes with every output file."""
+    out = run_cli(tmp_path, """rho 0.5
+cells_x 5
+cells_y 5
+cells_z 5
+type 2
+neigh_list 1.2 10
+dr_max 0.15
+npt 3.0 0.01
+opt 1 200 10 0.5 0.5
+press_thermo 0.00001 0.0002 10
+config_write 100 1
+seed 3
+sweep_eq 200
+sweep_stat 300
+out 100
+""")
+    assert "Simulation complete!" in out, out[-1500:]
+    dv = float([l for l in out.splitlines() if l.startswith("Optimal maximum volume deformation:")][0]
+               .split(":")[1])
+    assert math.isfinite(dv) and dv > 0.0
+    for name in ("density.dat", "press_thermo.dat", "config_000002.dat.gz"):
+        assert (tmp_path / name).exists(), name
+
+
 def test_checkerboard_npt_matches_serial_density(tmp_path):
     """NpT at beta P = 3 (N = 256): mean density of the checkerboard sweep
     followed by Binomial(N, 1/(N+1)) volume moves against the reference's
