@@ -1,6 +1,8 @@
 """Post-processing of the output files: compressibility factor from the
 contact value of g(r), thermodynamic-route pressure, Widom chemical
-potential, NpT density, averaged g(r), and the blocking / jackknife error
+potential, NpT density (average, histogram, autocorrelation), averaged g(r)
+with its blocking analysis and finite-size correction, the direct
+correlation and bridge functions, and the blocking / jackknife error
 estimates.  These restate the reference's analysis scripts (fedluc/HSMC
 python/hsmc_pressure.py, hsmc_chem_pot.py, hsmc_stat.py, hsmc_density.py,
 hsmc_rdf.py) in numpy; the reference's scripts are not used.
@@ -179,7 +181,197 @@ def average_rdf(samples: list) -> tuple:
     return samples[0].x, g.mean(axis=0), g.std(axis=0, ddof=1) / math.sqrt(len(g))
 
 
+def rdf_target_bins(r, target_r) -> list:
+    """Index of the first bin whose r is >= each target distance, the rule of
+    hsmc_rdf.py:100-111; a target beyond the last bin is an error."""
+    r = np.asarray(r, dtype=np.float64)
+    out = []
+    for t in target_r:
+        hit = np.nonzero(r >= t)[0]
+        if len(hit) == 0:
+            raise ValueError(f"target distance {t:.8f} lies beyond the last g(r) bin")
+        out.append(int(hit[0]))
+    return out
+
+
+def rdf_blocking(samples: list, target_r) -> list:
+    """Blocking analysis of g at chosen distances over all samples
+    (hsmc_rdf.py:75-155): each target snaps to the first bin at or beyond
+    it, and the sample series of that bin goes through blocking_std.
+    Returns [(r_bin, blocking_std array (levels, 2)), ...]."""
+    idx = rdf_target_bins(samples[0].x, target_r)
+    g = np.array([s.y for s in samples])
+    return [(float(samples[0].x[i]), blocking_std(g[:, i])) for i in idx]
+
+
+def write_rdf_average(samples: list, out_dir: str = ".") -> str:
+    """The averaged g(r) file of hsmc_rdf.py:59-70: two columns (r, mean g)
+    at %.16e, named rdf_average_<samples>config.dat.  Returns the path."""
+    import os
+    r, g, _ = average_rdf(samples)
+    path = os.path.join(out_dir, "rdf_average_%dconfig.dat" % len(samples))
+    np.savetxt(path, np.column_stack([r, g]), fmt="%.16e")
+    return path
+
+
+def write_rdf_blocking(samples: list, target_r, source: str = ".",
+                       path: str = "rdf_blocking_analysis.dat") -> str:
+    """The blocking report of hsmc_rdf.py:123-155: a header (source, total
+    samples, the snapped target distances at %.8f), then per target the
+    blocking levels as 'level sigma uncertainty'.  Returns the path."""
+    res = rdf_blocking(samples, target_r)
+    with open(path, "w") as f:
+        f.write("Blocking analysis of the radial distribution function data in " + source + "\n")
+        f.write("Total number of samples: %d\n" % len(samples))
+        f.write("Target positions:\n")
+        for r, _ in res:
+            f.write("%.8f\n" % r)
+        for r, sig in res:
+            f.write("###############################################\n")
+            f.write("Blocking analysis for target distance %.8f\n" % r)
+            f.write("Transformation, variance, variance uncertainty\n")
+            for k in range(len(sig)):
+                f.write("%d %.8f %.8f\n" % (k, sig[k, 0], sig[k, 1]))
+    return path
+
+
+def finite_size_correction(g, rho: float, n_part: int) -> np.ndarray:
+    """g(r) of an N-particle canonical run scaled by 1 + rho / (N beta dP/drho),
+    with beta dP/drho from the Pade equation of state below
+    (hsmc_rdf.py:318-326, which writes the factor as 1 + 1/(N mu_T) with
+    mu_T = (beta dP/drho)/rho).  The Lebowitz-Percus tail g_N -> 1 - S(0)/N,
+    S(0) = 1/(beta dP/drho), would give 1 + S(0)/N, smaller by the factor
+    rho; the reference's factor is kept."""
+    iic = inverse_compressibility_ew(math.pi * rho / 6.0)
+    return np.asarray(g, dtype=np.float64) * (1.0 + rho / (n_part * iic))
+
+
+def _radial_grid(r, g):
+    """Midpoint grid (k + 1/2) dr from 0 to the last g(r) bin, with h = g - 1
+    on the bins and h = -1 inside the core below the first bin
+    (hsmc_rdf.py:224-236)."""
+    r = np.asarray(r, dtype=np.float64)
+    dr = r[1] - r[0]
+    n0 = int(round(r[0] / dr - 0.5))
+    if n0 < 0 or abs((n0 + 0.5) * dr - r[0]) > 1e-6 * dr:
+        raise ValueError("g(r) bins must be centred on the grid (k + 1/2) dr")
+    rr = (np.arange(n0 + len(r)) + 0.5) * dr
+    h = np.full(len(rr), -1.0)
+    h[n0:] = np.asarray(g, dtype=np.float64) - 1.0
+    return rr, h, dr
+
+
+def direct_correlation(r, g, rho: float, n_part: int) -> dict:
+    """Direct correlation function c(r) = h(r) - gamma(r) from the
+    Ornstein-Zernike relation (hsmc_rdf.py:209-265).  g is first scaled by
+    finite_size_correction.  The radial Fourier transform
+    f(q) = (4 pi / q) int r f(r) sin(q r) dr is a midpoint sum, a type-II
+    sine transform on the grid r_k = (k + 1/2) dr, q_j = (j + 1) pi/(n dr);
+    gamma(q) = rho h(q)^2 / (1 + rho h(q)) goes back through the type-III
+    transform, gamma(r) = (dq / (2 pi^2 r)) sum q gamma(q) sin(q r).
+    Returns r, c, gamma, h, the corrected g and S(q) = 1 + rho h(q) on q."""
+    from scipy.fft import dst
+    gc = finite_size_correction(g, rho, n_part)
+    rr, h, dr = _radial_grid(r, gc)
+    n = len(rr)
+    dq = math.pi / (n * dr)
+    q = (np.arange(n) + 1.0) * dq
+    hq = 4.0 * math.pi * dr / q * dst(rr * h, type=2) / 2.0
+    gq = rho * hq * hq / (1.0 + rho * hq)
+    gam = dq / (2.0 * math.pi ** 2 * rr) * dst(q * gq, type=3) / 2.0
+    return {"r": rr, "c": h - gam, "gamma": gam, "h": h, "g": gc,
+            "q": q, "S": 1.0 + rho * hq}
+
+
+def bridge_function(r, g, rho: float, n_part: int, ln_y=None) -> dict:
+    """Bridge function B(r) = ln y(r) - gamma(r) (hsmc_rdf.py:268-316), with
+    ln y = ln g outside the core and NaN inside unless the cavity function
+    ln y is given on the same grid."""
+    d = direct_correlation(r, g, rho, n_part)
+    rr = d["r"]
+    if ln_y is None:
+        ly = np.full(len(rr), np.nan)
+        n0 = len(rr) - len(d["g"])
+        with np.errstate(divide="ignore"):
+            ly[n0:] = np.log(d["g"])
+    else:
+        ly = np.asarray(ln_y, dtype=np.float64)
+        if ly.shape != rr.shape:
+            raise ValueError("ln y must be given on the grid (k + 1/2) dr from 0")
+    return {"r": rr, "B": ly - d["gamma"], "ln_y": ly, "c": d["c"], "gamma": d["gamma"]}
+
+
+# ------------------------------------------------------------ density series
+
+def density_histogram(series, bins: int = 100) -> tuple:
+    """Histogram of the NpT density samples (hsmc_density.py:42-58)."""
+    return np.histogram(np.asarray(series, dtype=np.float64).ravel(), bins=bins)
+
+
+def acf(data, norm: bool = True) -> np.ndarray:
+    """Autocorrelation by direct summation (hsmc_stat.py:69-88):
+    c_k = sum_{j < n-k} (x_j - m)(x_{j+k} - m) / n, optionally over c_0."""
+    x = np.asarray(data, dtype=np.float64)
+    n = len(x)
+    d = x - x.mean()
+    c = np.array([np.dot(d[: n - k], d[k:]) / n for k in range(n)])
+    return c / c[0] if norm else c
+
+
+def acf_fft(data, norm: bool = True) -> np.ndarray:
+    """The same autocorrelation through an FFT zero-padded to twice the next
+    power of two, so the circular product has no wrap-around
+    (hsmc_stat.py:91-103)."""
+    x = np.asarray(data, dtype=np.float64)
+    n = len(x)
+    m = 1 << max(0, (n - 1).bit_length())
+    f = np.fft.fft(x - x.mean(), n=2 * m)
+    c = np.fft.ifft(f * np.conjugate(f))[:n].real / n
+    return c / c[0] if norm else c
+
+
+def density_acf(series, samples_block: int = -1, mode: str = "fft") -> np.ndarray:
+    """Autocorrelation of the density series averaged over blocks of
+    samples_block samples (the trailing remainder dropped; -1 or more than
+    the series: one block), as hsmc_density.py:73-100 and :127-137."""
+    x = np.asarray(series, dtype=np.float64).ravel()
+    if samples_block < 0 or samples_block > len(x):
+        samples_block = len(x)
+    blocks = x[: len(x) - len(x) % samples_block].reshape(-1, samples_block)
+    fn = {"fft": acf_fft, "dsum": acf}.get(mode)
+    if fn is None:
+        raise ValueError(f"unknown autocorrelation mode {mode!r}")
+    return np.mean([fn(b) for b in blocks], axis=0)
+
+
 # --------------------------------------------------------- closed-form anchors
+
+# The Pade form of the hard-sphere fluid equation of state that the reference
+# attributes to Erpenbeck and Wood, Z - 1 = eta P(eta) / Q(eta), with its
+# coefficients (hsmc_rdf.py:333-339).  Its Taylor series reproduces the virial
+# coefficients 4, 10 and 18.36; at eta = 0.25, 0.40 and 0.45 it lies 0.3%, 1.2%
+# and 1.8% above the Monte Carlo pressures of De Miguel and Jackson.
+_EW_P = (4.0, 0.890851, 0.8924486, 0.3430298)
+_EW_Q = (1.0, -2.277287, 1.3262418)
+
+
+def compressibility_ew(eta: float) -> float:
+    """Z = beta P / rho of that Pade form."""
+    p = _EW_P[0] + eta * (_EW_P[1] + eta * (_EW_P[2] + eta * _EW_P[3]))
+    q = _EW_Q[0] + eta * (_EW_Q[1] + eta * _EW_Q[2])
+    return 1.0 + eta * p / q
+
+
+def inverse_compressibility_ew(eta: float) -> float:
+    """beta dP/drho = d(rho Z)/drho = Z + eta dZ/deta for that Pade form
+    (hsmc_rdf.py:329-352)."""
+    p = _EW_P[0] + eta * (_EW_P[1] + eta * (_EW_P[2] + eta * _EW_P[3]))
+    dp = _EW_P[1] + eta * (2.0 * _EW_P[2] + eta * 3.0 * _EW_P[3])
+    q = _EW_Q[0] + eta * (_EW_Q[1] + eta * _EW_Q[2])
+    dq = _EW_Q[1] + 2.0 * eta * _EW_Q[2]
+    dz = p / q + eta * (dp * q - p * dq) / (q * q)   # d(eta P/Q)/d eta
+    return 1.0 + eta * p / q + eta * dz
+
 
 def carnahan_starling(rho: float) -> dict:
     """Carnahan-Starling Z, contact value and excess chemical potential."""
