@@ -439,17 +439,18 @@ def point_descriptors(ctx: "Context", xyz, r: int = 0) -> np.ndarray:
     return out
 
 
-def lattice(type_: int, n: int, rho: float):
+def lattice(type_: int, n, rho: float):
     """FCC (type 2) or SC (type 1) starting lattice and box, as sim_box_init
-    and part_init build them (sim_info.c:32-71, :125-156)."""
+    and part_init build them (sim_info.c:32-71, :125-156).  n is the number
+    of unit cells per edge, or a tuple (cells_x, cells_y, cells_z)."""
     import math
+    nx, ny, nz = (n, n, n) if isinstance(n, int) else tuple(int(v) for v in n)
     pc = 1 if type_ == 1 else 4
     a = math.pow(pc / rho, 1.0 / 3.0)
-    L = n * a
     pts = []
-    for i in range(n):
-        for j in range(n):
-            for k in range(n):
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
                 if type_ == 1:
                     pts.append((i * a, j * a, k * a))
                 else:
@@ -460,4 +461,4 @@ def lattice(type_: int, n: int, rho: float):
     conf = np.zeros((len(pts), 4))
     conf[:, 0] = np.arange(len(pts))
     conf[:, 1:] = np.array(pts)
-    return conf, (L, L, L)
+    return conf, (nx * a, ny * a, nz * a)

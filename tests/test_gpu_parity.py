@@ -266,3 +266,69 @@ def test_widom_hits_match_oracle_at_n108000():
     assert np.array_equal(got, o.widom_points(edge))
     assert got[:2000].sum() == 0   # inside a sphere: never insertable
     ctx.close()
+
+
+def _min_image_s2(a, b, box):
+    """Squared minimum-image distances (one single wrap per axis, the
+    reference's compute_dist order) between rows of a and rows of b."""
+    s = 0.0
+    for k in range(3):
+        d = a[:, None, k] - b[None, :, k]
+        L = box[k]
+        d = np.where(d > L / 2, d - L, np.where(d < -L / 2, d + L, d))
+        s = s + d * d
+    return s
+
+
+def test_anisotropic_box_sweeps_and_observables():
+    """fcc 6 x 4 x 5 cells at rho = 0.7: three different box edges and sweep
+    grid sizes.  The reference's own cell index is right only for cubes, so
+    the checks are the oracle's restatement of the sweep (bit for bit, alone
+    and batched beside a cubic replica) and brute force with the minimum
+    image taken per axis (pair histogram on a 3 x 2 x 2 grid, Widom hits)."""
+    lat, box = gpu.lattice(2, (6, 4, 5), 0.7)
+    assert len(lat) == 480 and abs(box[0] * box[1] * box[2] - 480 / 0.7) < 1e-9
+    g = tuple(int(L) // 2 * 2 for L in box)   # the even sweep grid, cells >= 1
+    assert len(set(g)) == 3
+    start = O.Checkerboard(lat, box, g, 0.4, 99)
+    start.sweeps(200)                          # a melted start, with cavities
+    conf = start.config()
+    rep = dict(n=len(conf), box=box, dr_max=0.12, seed=17)
+    ctx = gpu.Context([rep])
+    ctx.upload(conf)
+    assert ctx.grid() == g
+    cb = O.Checkerboard(conf, box, g, 0.12, 17)
+    ctx.sweeps(6)
+    cb.sweeps(6)
+    out = ctx.download()
+    assert np.array_equal(out, cb.config())
+    c = ctx.counters()
+    assert (c["attempts"], c["accepted"], c["left_cell"], c["sweeps"]) == \
+        tuple(int(v) for v in cb.counters())
+
+    cube, cbox = gpu.lattice(2, 5, 0.6)
+    pair = gpu.Context([dict(n=len(cube), box=cbox, dr_max=0.1, seed=3), rep])
+    pair.upload(cube, 0)
+    pair.upload(conf, 1)
+    pair.sweeps(6)
+    assert np.array_equal(pair.download(1), out)
+
+    p = out[:, 1:]
+    s = _min_image_s2(p, p, box)[np.triu_indices(len(p), 1)]
+    assert s.min() >= 1.0
+    dr, nb = 0.01, 200
+    rmax = 1.0 + nb * dr                       # 3.0 < half the shortest edge
+    assert rmax < min(box) / 2
+    r = np.sqrt(s[s < rmax * rmax])
+    r = r[r < rmax]
+    bins = ((r - 1.0) / dr).astype(np.int64)
+    cnt, oor = ctx.rdf_counts(dr, nb, rmax)
+    assert np.array_equal(cnt, np.bincount(bins[bins < nb], minlength=nb))
+    assert oor == int((bins >= nb).sum())
+
+    q = np.random.default_rng(5).random((100000, 3)) * np.array(box)
+    hits = ctx.widom_points(q)
+    ref = np.concatenate([~(_min_image_s2(q[i:i + 10000], p, box) < 1.0).any(axis=1)
+                          for i in range(0, len(q), 10000)])
+    assert np.array_equal(hits.astype(bool), ref)
+    assert ref.sum() > 20
