@@ -142,13 +142,15 @@ k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__res
   // is held by lane e % LPC as its copy e / LPC.  LDS holds the slot of each
   // entry and the particle id and id order of the active cell's occupants.
   constexpr int M = (LCAP + LPC - 1) / LPC;
-  extern __shared__ int smem_i[];
+  extern __shared__ __attribute__((aligned(16))) int smem_i[];
   const int cpb = blockDim.x / LPC;
   const int lane = threadIdx.x & (LPC - 1);
   const int sub = threadIdx.x / LPC;
-  int *Lm = smem_i + (size_t)sub * (LCAP + 2 * CAP);  // slot of each list entry
+  int *Lm = smem_i + (size_t)sub * (LCAP + 14 * CAP);  // slot of each list entry
   int *Sp = Lm + LCAP;                                 // ids of the active cell's occupants
   int *Lo = Sp + CAP;                                  // occupants in ascending id order
+  double *Op = reinterpret_cast<double *>(Lo + CAP);   // occupants' positions (x, y, z)
+  double *Od = Op + 3 * CAP;                           // each attempt's displacement
 
   const int a = blockIdx.x * cpb + sub;
   if (a >= A) return;  // the whole lane group leaves together
@@ -248,7 +250,15 @@ k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__res
       pz[j] = rec->z;
     }
   }
-  if (lane < nc) Sp[lane] = slot[Lm[oc + lane]].id;
+  if (lane < nc) {
+    const SlotRec *rec = slot + Lm[oc + lane];
+    Sp[lane] = rec->id;
+    if constexpr (LPC != 64) {
+      Op[3 * lane] = rec->x;
+      Op[3 * lane + 1] = rec->y;
+      Op[3 * lane + 2] = rec->z;
+    }
+  }
   wave_lds_sync();
   // occupants of the active cell in ascending particle id; attempt t moves
   // list entry oc + Lo[k_t]
@@ -259,51 +269,63 @@ k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__res
     Lo[rank] = lane;
   }
   wave_lds_sync();
-  const int my_e = lane < nc ? oc + Lo[my_k] : 0;
+  // 32-lane groups: lane t < nc prepares attempt t before the serial loop
+  // (its sphere's position, the trial position, whether it stays in the
+  // cell), so each attempt is one broadcast.  Whole-wave groups broadcast
+  // with scalar reads, cheap enough to fetch the sphere at each attempt.
+  int my_e = 0, my_stay = 0;
+  double my_nx = 0.0, my_ny = 0.0, my_nz = 0.0;
+  auto trial = [&](double bx, double by, double bz, double dx, double dy, double dz) {
+    my_nx = wrap_once(bx + dx, rp.lx);
+    my_ny = wrap_once(by + dy, rp.ly);
+    my_nz = wrap_once(bz + dz, rp.lz);
+    // all three axes are evaluated (no short circuit)
+    my_stay = (int)(shifted_cell(my_nx, shx, rp.lx, rp.ix, rp.gx) == cx) &
+              (int)(shifted_cell(my_ny, shy, rp.ly, rp.iy, rp.gy) == cy) &
+              (int)(shifted_cell(my_nz, shz, rp.lz, rp.iz, rp.gz) == cz);
+  };
+  if (lane < nc) {
+    const int o = Lo[my_k];
+    my_e = oc + o;
+    if constexpr (LPC != 64) {
+      Od[3 * lane] = my_dx;  // kept for a repeated pick, off the registers
+      Od[3 * lane + 1] = my_dy;
+      Od[3 * lane + 2] = my_dz;
+      trial(Op[3 * o], Op[3 * o + 1], Op[3 * o + 2], my_dx, my_dy, my_dz);
+    }
+  }
 
   unsigned int n_acc = 0, n_left = 0;
   unsigned int moved = 0u;  // bit j: this lane's copy j was moved
   for (int t = 0; t < nc; ++t) {
-    // attempt t of each lane group: its draw, then the moving sphere's
-    // current position from the lane holding it
     unsigned e;
-    double ddx, ddy, ddz, ex, ey, ez;
-    int owner, je;
+    double nx, ny, nz;
+    int stays;
     if constexpr (LPC == 64) {
+      // attempt t: its draw, then the moving sphere's current position
+      // from the lane holding it
       e = (unsigned)__builtin_amdgcn_readlane(my_e, t);
-      ddx = wave_lane(my_dx, t);
-      ddy = wave_lane(my_dy, t);
-      ddz = wave_lane(my_dz, t);
-      owner = (int)(e % LPC);
-      je = (int)(e / LPC);
-      ex = wave_lane((double)px[je], owner);
-      ey = wave_lane((double)py[je], owner);
-      ez = wave_lane((double)pz[je], owner);
+      const double ddx = wave_lane(my_dx, t), ddy = wave_lane(my_dy, t), ddz = wave_lane(my_dz, t);
+      const int o0 = (int)(e % LPC), j0 = (int)(e / LPC);
+      nx = wrap_once(wave_lane((double)px[j0], o0) + ddx, rp.lx);
+      ny = wrap_once(wave_lane((double)py[j0], o0) + ddy, rp.ly);
+      nz = wrap_once(wave_lane((double)pz[j0], o0) + ddz, rp.lz);
+      stays = (int)(shifted_cell(nx, shx, rp.lx, rp.ix, rp.gx) == cx) &
+              (int)(shifted_cell(ny, shy, rp.ly, rp.iy, rp.gy) == cy) &
+              (int)(shifted_cell(nz, shz, rp.lz, rp.iz, rp.gz) == cz);
     } else {
+      // attempt t as lane t prepared it
       e = (unsigned)__shfl(my_e, t, LPC);
-      ddx = __shfl(my_dx, t, LPC);
-      ddy = __shfl(my_dy, t, LPC);
-      ddz = __shfl(my_dz, t, LPC);
-      owner = (int)(e % LPC);
-      je = (int)(e / LPC);
-      ex = __shfl((double)px[je], owner, LPC);
-      ey = __shfl((double)py[je], owner, LPC);
-      ez = __shfl((double)pz[je], owner, LPC);
+      stays = __shfl(my_stay, t, LPC);
+      nx = __shfl(my_nx, t, LPC);
+      ny = __shfl(my_ny, t, LPC);
+      nz = __shfl(my_nz, t, LPC);
     }
-    double nx = ex + ddx;
-    double ny = ey + ddy;
-    double nz = ez + ddz;
-    nx = wrap_once(nx, rp.lx);
-    ny = wrap_once(ny, rp.ly);
-    nz = wrap_once(nz, rp.lz);
-    // all three axes are evaluated (no short circuit), so the test is one branch
-    const bool stays = ((int)(shifted_cell(nx, shx, rp.lx, rp.ix, rp.gx) == cx) &
-                        (int)(shifted_cell(ny, shy, rp.ly, rp.iy, rp.gy) == cy) &
-                        (int)(shifted_cell(nz, shz, rp.lz, rp.iz, rp.gz) == cz)) != 0;
     if (!stays) {
       ++n_left;
       continue;
     }
+    const int owner = (int)(e % LPC), je = (int)(e / LPC);
     bool ov = false;
 #pragma unroll
     for (int j = 0; j < M; ++j) {
@@ -325,6 +347,11 @@ k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__res
         moved |= 1u << je;
       }
       ++n_acc;
+      if constexpr (LPC != 64) {
+        // a later attempt on the same sphere starts from its new position
+        if (lane > t && lane < nc && my_e == (int)e)
+          trial(nx, ny, nz, Od[3 * lane], Od[3 * lane + 1], Od[3 * lane + 2]);
+      }
     }
   }
   // write the spheres that moved back (slot copy and by-id arrays), through
@@ -483,7 +510,9 @@ int rebin_last_sweep(hsmc_gpu_ctx *ctx) {
 // per cell: the slot map of up to lcap stencil spheres and the id and id
 // order of the cell's own occupants
 size_t phase_lds_bytes(int cap, int lcap, int cpb) {
-  return (size_t)cpb * ((size_t)lcap + 2 * (size_t)cap) * sizeof(int);
+  // per cell: slot map and occupant ids and order (ints); occupant positions
+  // and attempt displacements (doubles)
+  return (size_t)cpb * (((size_t)lcap + 2 * (size_t)cap) * sizeof(int) + 6 * (size_t)cap * sizeof(double));
 }
 }  // namespace hsmc
 
