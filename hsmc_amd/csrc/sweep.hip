@@ -140,7 +140,8 @@ k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__res
   // The stencil list (the spheres of the 27 neighbour cells, at most LCAP, a
   // bound the host derives from the cell side) lives in registers: entry e
   // is held by lane e % LPC as its copy e / LPC.  LDS holds the slot of each
-  // entry and the particle id and id order of the active cell's occupants.
+  // entry, the particle ids, id order and positions of the active cell's
+  // occupants, and (32-lane groups) each attempt's displacement.
   constexpr int M = (LCAP + LPC - 1) / LPC;
   extern __shared__ __attribute__((aligned(16))) int smem_i[];
   const int cpb = blockDim.x / LPC;
