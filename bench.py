@@ -169,7 +169,7 @@ def main() -> None:
         ctx.set_tuning(*(int(v) for v in args.tuning.split(",")))
         tuning = ctx.get_tuning()
     else:
-        tuning = ctx.get_tuning() if args.no_autotune else ctx.autotune(64)
+        tuning = ctx.get_tuning() if args.no_autotune else ctx.autotune(512)
     print(f"[bench] tuning {tuning}", file=sys.stderr, flush=True)
 
     ctx.sweeps(args.warmup)
