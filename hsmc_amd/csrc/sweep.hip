@@ -234,6 +234,18 @@ k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__res
     my_dz = (u32_unit_closed(w[3]) - 0.5) * rp.dr_max;
   }
 
+  // the active cell's occupants first (slot t of the active cell for lane
+  // t < nc: no slot map needed), so their ranking and the attempt
+  // preparation below overlap the neighbour loads
+  int my_id = 0;
+  double o_x = 0.0, o_y = 0.0, o_z = 0.0;
+  if (lane < nc) {
+    const SlotRec *rec = slot + (size_t)lane * C + rp.coff + cid;
+    my_id = rec->id;
+    o_x = rec->x;
+    o_y = rec->y;
+    o_z = rec->z;
+  }
   // stage the stencil list: the slot of every entry into LDS, then each
   // lane loads its entries' FP64 positions into registers, all loads of the
   // work item in flight together
@@ -252,12 +264,11 @@ k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__res
     }
   }
   if (lane < nc) {
-    const SlotRec *rec = slot + Lm[oc + lane];
-    Sp[lane] = rec->id;
+    Sp[lane] = my_id;
     if constexpr (LPC != 64) {
-      Op[3 * lane] = rec->x;
-      Op[3 * lane + 1] = rec->y;
-      Op[3 * lane + 2] = rec->z;
+      Op[3 * lane] = o_x;
+      Op[3 * lane + 1] = o_y;
+      Op[3 * lane + 2] = o_z;
     }
   }
   wave_lds_sync();
