@@ -128,9 +128,10 @@ template <int M>
 using dvec = double __attribute__((ext_vector_type(M)));
 
 // Occupancy: at least 6 waves per SIMD (at most 80 VGPRs) for stencil lists
-// of up to 192 entries, 4 for the longest lists.
+// of up to 192 entries, 4 for the longest lists and for one replica (whose
+// few active cells never fill the device).
 template <int CAP, int LPC, int LCAP, bool ONE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LCAP <= 192 ? 6 : 4)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!ONE && LCAP <= 192 ? 6 : 4)))
 k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__restrict__ swp,
         const int *__restrict__ arep, int R, int A, int phase,
         int k, const unsigned long long *__restrict__ sweep_base, double *__restrict__ x,
@@ -296,9 +297,18 @@ k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__res
               (int)(shifted_cell(my_ny, shy, rp.ly, rp.iy, rp.gy) == cy) &
               (int)(shifted_cell(my_nz, shz, rp.lz, rp.iz, rp.gz) == cz);
   };
+  // one replica (few cells in flight, registers to spare): lane t also
+  // stores its sphere when attempt t is accepted, its slot record (slot o
+  // of the active cell) and by-id position, instead of a write-back after
+  // the loop
+  int my_slot = 0, my_g = 0;
   if (lane < nc) {
     const int o = Lo[my_k];
     my_e = oc + o;
+    if constexpr (ONE) {
+      my_slot = o * C + rp.coff + cid;  // below cap * C < 2^31
+      my_g = rp.poff + Sp[o];
+    }
     if constexpr (LPC != 64) {
       Od[3 * lane] = my_dx;  // kept for a repeated pick, off the registers
       Od[3 * lane + 1] = my_dy;
@@ -308,7 +318,7 @@ k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__res
   }
 
   unsigned int n_acc = 0, n_left = 0;
-  unsigned int moved = 0u;  // bit j: this lane's copy j was moved
+  unsigned int moved = 0u;  // several replicas: bit j, this lane's copy j moved
   for (int t = 0; t < nc; ++t) {
     unsigned e;
     double nx, ny, nz;
@@ -356,7 +366,19 @@ k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__res
         px[je] = nx;
         py[je] = ny;
         pz[je] = nz;
-        moved |= 1u << je;
+        if constexpr (!ONE) moved |= 1u << je;
+      }
+      // write the move through to the device-coherent level at once (no
+      // dirty L2 lines are left for the end-of-kernel write-back); a later
+      // move of the same sphere stores after it
+      if (ONE && lane == t) {
+        SlotRec *rec = slot + my_slot;
+        store_through(&rec->x, nx);
+        store_through(&rec->y, ny);
+        store_through(&rec->z, nz);
+        store_through(&x[my_g], nx);
+        store_through(&y[my_g], ny);
+        store_through(&z[my_g], nz);
       }
       ++n_acc;
       if constexpr (LPC != 64) {
@@ -366,21 +388,22 @@ k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__res
       }
     }
   }
-  // write the spheres that moved back (slot copy and by-id arrays), through
-  // to the device-coherent level so no dirty L2 lines are left for the
-  // end-of-kernel write-back
+  if constexpr (!ONE) {
+    // write the spheres that moved back (slot copy and by-id arrays),
+    // through to the device-coherent level
 #pragma unroll
-  for (int j = 0; j < M; ++j) {
-    if ((moved >> j) & 1u) {
-      const int q = lane + j * LPC;
-      SlotRec *rec = slot + Lm[q];
-      store_through(&rec->x, (double)px[j]);
-      store_through(&rec->y, (double)py[j]);
-      store_through(&rec->z, (double)pz[j]);
-      const int g = rp.poff + Sp[q - oc];
-      store_through(&x[g], (double)px[j]);
-      store_through(&y[g], (double)py[j]);
-      store_through(&z[g], (double)pz[j]);
+    for (int j = 0; j < M; ++j) {
+      if ((moved >> j) & 1u) {
+        const int q = lane + j * LPC;
+        SlotRec *rec = slot + Lm[q];
+        store_through(&rec->x, (double)px[j]);
+        store_through(&rec->y, (double)py[j]);
+        store_through(&rec->z, (double)pz[j]);
+        const int g = rp.poff + Sp[q - oc];
+        store_through(&x[g], (double)px[j]);
+        store_through(&y[g], (double)py[j]);
+        store_through(&z[g], (double)pz[j]);
+      }
     }
   }
   // move counters per active-cell index (the cells of a workgroup share a
