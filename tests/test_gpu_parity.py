@@ -332,3 +332,28 @@ def test_anisotropic_box_sweeps_and_observables():
                           for i in range(0, len(q), 10000)])
     assert np.array_equal(hits.astype(bool), ref)
     assert ref.sum() > 20
+
+
+@pytest.mark.parametrize("tuning", [(32, 2, 0), (64, 2, 0), (32, 1, 4)])
+def test_wide_cell_single_replica_matches_oracle(tuning):
+    """A dilute single replica (fcc 5^3 at rho = 0.3: sweep cells of 1.19
+    diameters, so 16 slots per cell and a 192-entry stencil list) with 32-
+    and 64-lane groups, direct and graph-replayed: bit for bit equal to the
+    oracle's restatement of the sweep, counters included."""
+    lat, box = gpu.lattice(2, 5, 0.3)
+    g = tuple(int(L) // 2 * 2 for L in box)
+    assert abs(box[0] / g[0] - 1.1856) < 1e-3
+    start = O.Checkerboard(lat, box, g, 0.5, 7)
+    start.sweeps(50)                           # a melted start
+    conf = start.config()
+    ctx = gpu.Context([dict(n=len(conf), box=box, dr_max=0.3, seed=23)])
+    ctx.set_tuning(*tuning)
+    ctx.upload(conf)
+    assert ctx.grid() == g
+    cb = O.Checkerboard(conf, box, g, 0.3, 23)
+    ctx.sweeps(12)
+    cb.sweeps(12)
+    assert np.array_equal(ctx.download(), cb.config())
+    c = ctx.counters()
+    assert (c["attempts"], c["accepted"], c["left_cell"], c["sweeps"]) == \
+        tuple(int(v) for v in cb.counters())
