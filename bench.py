@@ -289,7 +289,10 @@ def npt(args) -> None:
     for r, c in enumerate(confs):
         ctx.upload(c, r)
     del confs
-    print(f"[bench] npt: {R} replicas, {total} spheres", file=sys.stderr, flush=True)
+    if not args.tuning and not args.no_autotune:
+        ctx.autotune(16)  # NVT trials of the same phase kernels; positions are restored
+    print(f"[bench] npt: {R} replicas, {total} spheres, tuning {ctx.get_tuning()}",
+          file=sys.stderr, flush=True)
     ctx.npt_sweeps(args.warmup, press, dv)
     ctx.sync()
     ctx.reset_counters()
