@@ -1,77 +1,104 @@
 /*
- * exec.c -- the hsmc command line (exec.c:85-146 of the reference): -i IN_FILE
- * (default in.dat), -o OUT_FILE, -e; NpT when the input sets a pressure,
- * NVT otherwise.  The cavity and cluster ensembles are not part of this build.
+ * exec.c -- the `hsmc` executable: command-line handling and the choice of
+ * ensemble driver.
+ *
+ * Interface kept from the reference's CLI (exec.c:28-35 options, :122-137
+ * dispatch), so existing job scripts keep working:
+ *   -i, --input FILE    keyword file to read (default: in.dat)
+ *   -o, --output FILE   send the screen log to FILE
+ *   -e, --example       print a sample keyword file and stop
+ * The driver is hs_npt when the input sets a positive pressure, hs_nvt
+ * otherwise.  The cavity-biased and cluster-move ensembles belong to the
+ * reference only; an input that asks for them is refused.
  */
-#include <argp.h>
-#include <stdbool.h>
+#include <getopt.h>
+#include <stdio.h>
 #include <stdlib.h>
-#include <string.h>
 
 #include "hsmc.h"
 
-static char doc[] =
-    "hsmc performs Monte Carlo simulations of mono-disperse hard-sphere systems on an AMD GPU.\n"
-    "The program is run as ./hsmc -i IN_FILE where IN_FILE is a file describing\n"
-    "the parameters of the simulation. An example of input file can be obtained by\n"
-    "running ./hsmc -e.";
+enum cli_action { CLI_RUN, CLI_EXAMPLE, CLI_HELP };
 
-static struct argp_option options[] = {
-    {"input", 'i', "IN_FILE", 0, "Input read from IN_FILE instead of from  in.dat", 0},
-    {"example", 'e', 0, 0, "Print example of input file on screen", 0},
-    {"output", 'o', "OUT_FILE", 0, "Output to OUT_FILE instead of standard output", 0},
-    {0}};
-
-struct arguments {
-  char *input_file;
-  char *output_file;
-  bool example;
+struct cli_request {
+  enum cli_action action;
+  const char *keyword_file;
+  const char *log_file; /* NULL: keep stdout */
 };
 
-static error_t parse_opt(int key, char *arg, struct argp_state *state) {
-  struct arguments *a = state->input;
-  switch (key) {
-    case 'i': a->input_file = arg; break;
-    case 'o': a->output_file = arg; break;
-    case 'e': a->example = true; break;
-    case ARGP_KEY_ARG:
-      if (state->arg_num > 0) argp_usage(state);
-      break;
-    default: return ARGP_ERR_UNKNOWN;
+static void show_help(FILE *to, const char *prog) {
+  fprintf(to,
+          "usage: %s [-i FILE] [-o FILE] [-e] [-h]\n"
+          "Hard-sphere Metropolis Monte Carlo (NVT or NpT) on an MI355X GPU.\n"
+          "  -i, --input FILE    keyword file describing the run (default in.dat)\n"
+          "  -o, --output FILE   write the run log to FILE rather than the terminal\n"
+          "  -e, --example       print a sample keyword file, then exit\n"
+          "  -h, --help          show this text\n",
+          prog);
+}
+
+/* Fills *req from argv; returns 0, or 2 after printing a usage error. */
+static int read_command_line(int argc, char **argv, struct cli_request *req) {
+  static const struct option longopts[] = {
+      {"input", required_argument, NULL, 'i'},
+      {"output", required_argument, NULL, 'o'},
+      {"example", no_argument, NULL, 'e'},
+      {"help", no_argument, NULL, 'h'},
+      {NULL, 0, NULL, 0}};
+  req->action = CLI_RUN;
+  req->keyword_file = "in.dat";
+  req->log_file = NULL;
+  int opt;
+  while ((opt = getopt_long(argc, argv, "i:o:eh", longopts, NULL)) != -1) {
+    if (opt == 'i') {
+      req->keyword_file = optarg;
+    } else if (opt == 'o') {
+      req->log_file = optarg;
+    } else if (opt == 'e') {
+      if (req->action == CLI_RUN) req->action = CLI_EXAMPLE;
+    } else if (opt == 'h') {
+      req->action = CLI_HELP;
+    } else {
+      show_help(stderr, argv[0]);
+      return 2;
+    }
+  }
+  /* one stray operand is tolerated (the reference's parser allows it); more
+   * is a usage error */
+  if (argc - optind > 1) {
+    fprintf(stderr, "%s: unexpected operand '%s'\n", argv[0], argv[optind + 1]);
+    show_help(stderr, argv[0]);
+    return 2;
   }
   return 0;
 }
 
-static struct argp argp = {options, parse_opt, 0, doc, 0, 0, 0};
-
 int main(int argc, char **argv) {
-  struct arguments a;
-  bool run = true;
-  a.input_file = "in.dat";
-  a.output_file = "";
-  a.example = false;
-  argp_parse(&argp, argc, argv, 0, 0, &a);
-  if (a.example) {
-    print_example();
-    run = false;
+  struct cli_request req;
+  int st = read_command_line(argc, argv, &req);
+  if (st) return st;
+  if (req.action == CLI_HELP) {
+    show_help(stdout, argv[0]);
+    return 0;
   }
-  if (strcmp(a.output_file, "") != 0) {
-    if (!freopen(a.output_file, "w", stdout)) {
-      printf("Failed to pipe output to file %s\n", a.output_file);
-      exit(EXIT_FAILURE);
-    }
+  /* the sample goes to the terminal even with -o (the log file is still
+   * created, empty), as in the reference */
+  if (req.action == CLI_EXAMPLE) print_example();
+  if (req.log_file && !freopen(req.log_file, "w", stdout)) {
+    fprintf(stderr, "hsmc: cannot open log file %s for writing\n", req.log_file);
+    return EXIT_FAILURE;
   }
-  if (run) {
-    read_input_file(a.input_file);
-    if (G_IN.press > 0) {
-      hs_npt();
-    } else if (G_IN.cavity_pcav > 0 || G_IN.cluster_flag > 0) {
-      printf("ERROR: cavity and cluster simulations are not part of this build\n");
-      exit(EXIT_FAILURE);
-    } else {
-      hs_nvt();
-    }
-    printf("Simulation complete!\n");
+  if (req.action == CLI_EXAMPLE) return 0;
+
+  read_input_file((char *)req.keyword_file); /* the reference signature takes char * */
+  const int other_ensemble = G_IN.cavity_pcav > 0 || G_IN.cluster_flag > 0;
+  if (G_IN.press > 0) {
+    hs_npt();
+  } else if (other_ensemble) {
+    printf("ERROR: this build runs NVT and NpT only (cavity-biased and cluster runs are not provided)\n");
+    return EXIT_FAILURE;
+  } else {
+    hs_nvt();
   }
+  printf("Simulation complete!\n");
   return 0;
 }
