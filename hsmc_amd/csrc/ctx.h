@@ -98,6 +98,10 @@ struct hsmc_gpu_ctx {
   std::vector<unsigned long long> tot_acc, tot_left;
   unsigned long long sweep_at_reset = 0;
   unsigned int *d_err = nullptr;
+  // pinned host copy of the error bits, refreshed by an asynchronous copy
+  // after each batch of sweeps, so later batches are not launched onto a
+  // cell list that has already overflowed
+  volatile unsigned int *h_err = nullptr;
   hsmc_gpu_tuning tune{32, 8, 32, 0};
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -115,6 +119,10 @@ struct hsmc_gpu_ctx {
   double aux_cut = 0.0;
   unsigned long long aux_version = 0;
   int aux_n[3] = {0, 0, 0};
+  // owner stamp of the learned proposal's voxel-score and cumulative-mass
+  // scratch buffers: every call that writes them takes a new value, and a
+  // network's cached scores are valid only while the stamp is the one it took
+  unsigned long long score_stamp = 0;
   // neighbour grids of the learned proposal (descriptor grid, overlap grid)
   int ngrid_replica[2] = {-1, -1};
   unsigned long long ngrid_version[2] = {0, 0};

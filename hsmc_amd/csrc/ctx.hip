@@ -253,6 +253,7 @@ int check_device_errors(hsmc_gpu_ctx *ctx) {
   unsigned int e = 0;
   HSMC_HIP(ctx, hipMemcpyAsync(&e, ctx->d_err, sizeof(e), hipMemcpyDeviceToHost, ctx->stream));
   HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->h_err) *ctx->h_err = 0u;
   if (e) {
     unsigned int zero = 0;
     HSMC_HIP(ctx, hipMemcpy(ctx->d_err, &zero, sizeof(zero), hipMemcpyHostToDevice));
@@ -364,6 +365,13 @@ int hsmc_gpu_create(hsmc_gpu_ctx **out, int device, int n_replicas,
       hipMalloc(&ctx->d_sweep, sizeof(unsigned long long)) != hipSuccess ||
       hipMalloc(&ctx->d_err, sizeof(unsigned int)) != hipSuccess)
     return fail(HSMC_ERR_NO_MEMORY);
+  {
+    void *pinned = nullptr;
+    if (hipHostMalloc(&pinned, sizeof(unsigned int), hipHostMallocDefault) != hipSuccess)
+      return fail(HSMC_ERR_NO_MEMORY);
+    ctx->h_err = (volatile unsigned int *)pinned;
+    *ctx->h_err = 0u;
+  }
   if (hipMemcpy(ctx->d_prep, prep.data(), sizeof(int) * ptot, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(ctx->d_sweep, 0, sizeof(unsigned long long)) != hipSuccess ||
       hipMemset(ctx->d_swp, 0, sizeof(SweepDev) * n_replicas) != hipSuccess ||
@@ -389,6 +397,7 @@ int hsmc_gpu_destroy(hsmc_gpu_ctx *ctx) {
     if (b) (void)hipFree(b);
   for (int i = 0; i < hsmc_gpu_ctx::kScratch; ++i)
     if (ctx->scratch[i]) (void)hipFree(ctx->scratch[i]);
+  if (ctx->h_err) (void)hipHostFree((void *)ctx->h_err);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -44,6 +44,26 @@ __host__ __device__ __forceinline__ double u32_unit_closed(uint32_t u) {
   return (double)u / 4294967295.0;
 }
 
+// Occupant pick of the checkerboard sweep: rng_get_int's rule (rng.c:34-36,
+// gsl_rng_uniform_int) applied to the sweep's Philox stream.  `first` is word
+// 0 of attempt t's block (counter cell, t, sweep); k = first / scale with
+// scale = (2^32 - 1) / n, and a k >= n is drawn again from the block with
+// counter word 1 = t | (retry << 16), retry = 1, 2, ...  Uniform on [0, n)
+// for any n in [1, 2^32 - 1] and t < 2^16.
+__host__ __device__ __forceinline__ uint32_t occupant_pick(uint32_t first, uint32_t n,
+                                                           uint32_t cell, uint32_t t,
+                                                           unsigned long long sweep, uint32_t k0,
+                                                           uint32_t k1) {
+  const uint32_t scale = 0xFFFFFFFFu / n;
+  uint32_t k = first / scale;
+  for (uint32_t rr = 1; k >= n; ++rr) {
+    uint32_t w[4];
+    philox4x32_10(cell, t | (rr << 16), (uint32_t)sweep, (uint32_t)(sweep >> 32), k0, k1, w);
+    k = w[0] / scale;
+  }
+  return k;
+}
+
 // Minimum image of one component (moves.c:419-424): d - L above half a box,
 // d + L below minus half a box.  Both candidates are computed and one is
 // selected, so the compiler emits selects rather than branches.

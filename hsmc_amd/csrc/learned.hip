@@ -657,7 +657,7 @@ __device__ __forceinline__ bool widom_free(const NGrid &g1, double qx, double qy
 __global__ void __launch_bounds__(256)
 k_insert(NGrid g1, Vox vx, const double *__restrict__ cdf, double total, long n, double u0,
          const float *__restrict__ f, double fmax, double sumexp, double mix, uint32_t k0,
-         uint32_t k1, unsigned long long sample, double *__restrict__ acc,
+         uint32_t k1, unsigned long long sample, double *__restrict__ part,
          unsigned long long *__restrict__ hits) {
   __shared__ double rs[4];
   __shared__ unsigned long long rh[4];
@@ -699,13 +699,28 @@ k_insert(NGrid g1, Vox vx, const double *__restrict__ cdf, double total, long n,
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const double sw = rs[0] + rs[1] + rs[2] + rs[3];
+    // the block's weight goes to its own slot and is summed in a fixed order
+    // afterwards (k_sum_fixed), so an estimate never depends on block timing
+    part[blockIdx.x] = rs[0] + rs[1] + rs[2] + rs[3];
     const unsigned long long sh = rh[0] + rh[1] + rh[2] + rh[3];
-    if (sh) {
-      atomicAdd(acc, sw);
-      atomicAdd(hits, sh);
-    }
+    if (sh) atomicAdd(hits, sh);
   }
+}
+
+// out = sum of part[0..n) in an order fixed by n alone: 256 strided partial
+// sums, then a pairwise tree.
+__global__ void __launch_bounds__(256) k_sum_fixed(const double *__restrict__ part, long n,
+                                                   double *__restrict__ out) {
+  __shared__ double s[256];
+  double a = 0.0;
+  for (long i = threadIdx.x; i < n; i += 256) a += part[i];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
 }
 
 // single-block exclusive scan of up to a few thousand entries (+ total)
@@ -751,6 +766,10 @@ struct hsmc_gpu_mlp {
   double sc_voxel = 0.0, sc_fmax = 0.0, sc_sumexp = 0.0;
   double sc_total = 0.0;  // total of the cached cumulative proposal mass (mix)
   double sc_mix = -1.0;
+  // the context's score stamp when this network last wrote the shared score
+  // and cumulative-mass buffers; another writer (a second network, or
+  // voxel_scores) changes the context's stamp and so invalidates the cache
+  unsigned long long sc_stamp = 0;
 };
 
 namespace {
@@ -790,8 +809,8 @@ int grid_cells_for2(double L, double cut) {
 // scratch slots 24..39 belong to this file (observables.hip uses 0..12,
 // ctx.hip 21..22, sweep.hip 23): descriptor grid 24..26, overlap grid 27..29,
 // rows / scores 30, targets / block partials 31, counts 32, offsets 33,
-// sums 34, insertion list 35, scan partials 36..37.
-enum { L_GRID_A = 24, L_GRID_B = 27, L_X = 30, L_T = 31, L_CDF = 32, L_ACC = 34,
+// sums 34, per-block insertion weights 35, scan partials 36..37.
+enum { L_GRID_A = 24, L_GRID_B = 27, L_X = 30, L_T = 31, L_CDF = 32, L_ACC = 34, L_PART = 35,
        L_BSUM = 36, L_F = 38, L_FRED = 39 };
 
 int build_ngrid(hsmc_gpu_ctx *ctx, int replica, double cut, int slot0, NGrid &g) {
@@ -1063,9 +1082,12 @@ int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mi
   double *cdf = (double *)scratch(ctx, L_CDF, sizeof(double) * (size_t)vx.V);
   double *bsum = (double *)scratch(ctx, L_BSUM, sizeof(double) * (size_t)(ncdf + 1));
   double *acc = (double *)scratch(ctx, L_ACC, sizeof(double) * 2);
-  if (!f || !cdf || !bsum || !acc) return HSMC_ERR_NO_MEMORY;
+  const long nblk_ins = (long)((n + 255) / 256);
+  double *part = (double *)scratch(ctx, L_PART, sizeof(double) * (size_t)nblk_ins);
+  if (!f || !cdf || !bsum || !acc || !part) return HSMC_ERR_NO_MEMORY;
   const bool scores_ok = m->sc_replica == replica && m->sc_version == ctx->version &&
-                         m->sc_step == m->step && m->sc_voxel == voxel;
+                         m->sc_step == m->step && m->sc_voxel == voxel &&
+                         m->sc_stamp == ctx->score_stamp;
   if (!scores_ok) {
     float *bred = nullptr;
     long nblk = 0;
@@ -1085,6 +1107,7 @@ int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mi
     m->sc_fmax = fm;
     m->sc_sumexp = se;
     m->sc_mix = -1.0;  // the cumulative mass is rebuilt below
+    m->sc_stamp = ++ctx->score_stamp;
   }
   const double fmax = m->sc_fmax, sumexp = m->sc_sumexp;
   if (m->sc_mix != mix) {
@@ -1099,6 +1122,7 @@ int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mi
     HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     m->sc_total = total;
     m->sc_mix = mix;
+    m->sc_stamp = ++ctx->score_stamp;
   }
   // one uniform offset per call for the systematic sample
   uint32_t u[4];
@@ -1106,10 +1130,11 @@ int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mi
                 (uint32_t)seed, (uint32_t)(seed >> 32), u);
   const double u0 = (double)u[0] * 0x1p-32;
   HSMC_HIP(ctx, hipMemsetAsync(acc, 0, sizeof(double) * 2, ctx->stream));
-  hipLaunchKernelGGL(k_insert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, g1, vx,
+  hipLaunchKernelGGL(k_insert, dim3((unsigned)nblk_ins), dim3(256), 0, ctx->stream, g1, vx,
                      cdf, m->sc_total, (long)n, u0, f, fmax, sumexp, mix, (uint32_t)seed,
-                     (uint32_t)(seed >> 32), (unsigned long long)sample, acc,
+                     (uint32_t)(seed >> 32), (unsigned long long)sample, part,
                      reinterpret_cast<unsigned long long *>(acc + 1));
+  hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(256), 0, ctx->stream, part, nblk_ins, acc);
   HSMC_HIP(ctx, hipGetLastError());
   double ha[2] = {0, 0};
   HSMC_HIP(ctx, hipMemcpyAsync(ha, acc, sizeof(double) * 2, hipMemcpyDeviceToHost, ctx->stream));
@@ -1134,6 +1159,8 @@ int hsmc_gpu_voxel_scores(hsmc_gpu_mlp *m, int replica, double voxel, float *sco
   if (st) return st;
   float *f = (float *)scratch(ctx, L_F, sizeof(float) * (size_t)vx.V);
   if (!f) return HSMC_ERR_NO_MEMORY;
+  // the shared score buffer is overwritten: no network's cache stays valid
+  ++ctx->score_stamp;
   m->sc_replica = -1;
   float *bred = nullptr;
   long nblk = 0;

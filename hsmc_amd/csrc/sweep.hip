@@ -221,15 +221,7 @@ k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__res
   uint32_t my_k = 0;
   double my_dx = 0.0, my_dy = 0.0, my_dz = 0.0;
   if (lane < nc) {
-    const uint32_t scale = 0xFFFFFFFFu / (uint32_t)nc;
-    uint32_t kk = w[0] / scale;
-    for (uint32_t rr = 1; kk >= (uint32_t)nc; ++rr) {
-      uint32_t w2[4];
-      philox4x32_10((uint32_t)cid, (uint32_t)lane | (rr << 16), (uint32_t)sweep,
-                    (uint32_t)(sweep >> 32), rp.k0, rp.k1, w2);
-      kk = w2[0] / scale;
-    }
-    my_k = kk;
+    my_k = occupant_pick(w[0], (uint32_t)nc, (uint32_t)cid, (uint32_t)lane, sweep, rp.k0, rp.k1);
     my_dx = (u32_unit_closed(w[1]) - 0.5) * rp.dr_max;
     my_dy = (u32_unit_closed(w[2]) - 0.5) * rp.dr_max;
     my_dz = (u32_unit_closed(w[3]) - 0.5) * rp.dr_max;
@@ -418,6 +410,17 @@ __global__ void k_advance(unsigned long long *sweep_base, int n) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *sweep_base += (unsigned long long)n;
 }
 
+// The phase kernel's occupant pick for attempts t < count of one cell, any n.
+__global__ void __launch_bounds__(256)
+k_occupant_picks(uint32_t k0, uint32_t k1, uint32_t cell, unsigned long long sweep, uint32_t n,
+                 int count, uint32_t *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  uint32_t w[4];
+  philox4x32_10(cell, (uint32_t)t, (uint32_t)sweep, (uint32_t)(sweep >> 32), k0, k1, w);
+  out[t] = occupant_pick(w[0], n, cell, (uint32_t)t, sweep, k0, k1);
+}
+
 using PhaseFn = void (*)(const RepDev *, const RepDev, const SweepDev *, const int *, int, int, int, int,
                          const unsigned long long *, double *, double *, double *, const int *,
                          int, SlotRec *, unsigned int *, unsigned int *, unsigned int *);
@@ -500,8 +503,14 @@ int capture_graph(hsmc_gpu_ctx *ctx, int K) {
 }  // namespace
 
 namespace hsmc {
+// After each batch of sweeps the device's error bits are copied to pinned
+// host memory without waiting; a batch is launched only while the copies
+// that have landed show no error (a cell overflow or a full candidate list
+// leaves sweeps running on an incomplete cell list).  The next
+// synchronising call reports the error.
 int run_sweeps(hsmc_gpu_ctx *ctx, int n) {
   if (n <= 0) return HSMC_OK;
+  if (*ctx->h_err) return check_device_errors(ctx);
   ctx->version++;
   const int K = ctx->tune.sweeps_per_graph;
   int done = 0;
@@ -511,13 +520,20 @@ int run_sweeps(hsmc_gpu_ctx *ctx, int n) {
       int st = capture_graph(ctx, K);
       if (st) return st;
     }
-    for (; done + K <= n; done += K) HSMC_HIP(ctx, hipGraphLaunch(ctx->gexec, ctx->stream));
+    for (; done + K <= n; done += K) {
+      if (*ctx->h_err) return check_device_errors(ctx);
+      HSMC_HIP(ctx, hipGraphLaunch(ctx->gexec, ctx->stream));
+      HSMC_HIP(ctx, hipMemcpyAsync((void *)ctx->h_err, ctx->d_err, sizeof(unsigned int),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    }
   }
   const int rem = n - done;
   if (rem > 0) {
     for (int k = 0; k < rem; ++k) enqueue_sweep(ctx, k, ctx->stream, nullptr);
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, ctx->stream, ctx->d_sweep, rem);
     HSMC_HIP(ctx, hipGetLastError());
+    HSMC_HIP(ctx, hipMemcpyAsync((void *)ctx->h_err, ctx->d_err, sizeof(unsigned int),
+                                 hipMemcpyDeviceToHost, ctx->stream));
   }
   ctx->cells_version = ctx->version;  // the last sweep's cells hold every sphere
   return HSMC_OK;
@@ -599,6 +615,26 @@ int hsmc_gpu_time_phase_kernel(hsmc_gpu_ctx *ctx, int n_sweeps, float *avg_ms, i
   *avg_ms = (float)(total / count);
   if (launches) *launches = count;
   return check_device_errors(ctx);
+}
+
+int hsmc_gpu_occupant_picks(int device, uint64_t seed, uint32_t cell, uint64_t sweep, uint32_t n,
+                            int32_t count, uint32_t *out) {
+  if (n == 0 || count < 0 || count > 65536 || (count > 0 && !out)) return HSMC_ERR_BAD_ARG;
+  if (count == 0) return HSMC_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return HSMC_ERR_NO_DEVICE;
+  if (device < 0 || device >= ndev) return HSMC_ERR_BAD_ARG;
+  if (hipSetDevice(device) != hipSuccess) return HSMC_ERR_HIP;
+  uint32_t *d = nullptr;
+  if (hipMalloc(&d, sizeof(uint32_t) * (size_t)count) != hipSuccess) return HSMC_ERR_NO_MEMORY;
+  hipLaunchKernelGGL(k_occupant_picks, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, 0,
+                     (uint32_t)seed, (uint32_t)(seed >> 32), cell, (unsigned long long)sweep, n,
+                     (int)count, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpy(out, d, sizeof(uint32_t) * (size_t)count, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return e == hipSuccess ? HSMC_OK : HSMC_ERR_HIP;
 }
 
 int hsmc_gpu_counters(hsmc_gpu_ctx *ctx, int replica, uint64_t out[4]) {

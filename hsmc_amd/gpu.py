@@ -32,7 +32,7 @@ EXPORTS = [
     "hsmc_gpu_cell_list", "hsmc_gpu_overlap_batch", "hsmc_gpu_overlap_scaled",
     "hsmc_gpu_any_overlap_scaled", "hsmc_gpu_pair_dist", "hsmc_gpu_rdf_counts",
     "hsmc_gpu_contact_counts", "hsmc_gpu_widom_points", "hsmc_gpu_widom_uniform",
-    "hsmc_gpu_set_particles", "hsmc_gpu_set_dr_max",
+    "hsmc_gpu_set_particles", "hsmc_gpu_set_dr_max", "hsmc_gpu_occupant_picks",
     "hsmc_gpu_mlp_create", "hsmc_gpu_mlp_destroy", "hsmc_gpu_mlp_set", "hsmc_gpu_mlp_get",
     "hsmc_gpu_mlp_forward", "hsmc_gpu_mlp_gradient", "hsmc_gpu_point_descriptors",
     "hsmc_gpu_mlp_train", "hsmc_gpu_voxel_scores", "hsmc_gpu_widom_learned",
@@ -124,6 +124,8 @@ def load() -> C.CDLL:
         "hsmc_gpu_voxel_scores": (i32, [vp, i32, f64, P(C.c_float), i64, P(i64)]),
         "hsmc_gpu_widom_learned": (i32, [vp, i32, f64, f64, u64, u64, i64, P(f64), P(i64),
                                          P(i64)]),
+        "hsmc_gpu_occupant_picks": (i32, [i32, u64, C.c_uint32, u64, C.c_uint32, i32,
+                                          P(C.c_uint32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -429,6 +431,18 @@ class Mlp:
                                                    int(sample), int(n), C.byref(est),
                                                    C.byref(done), C.byref(hits)), "widom_learned")
         return dict(estimate=est.value, n=done.value, hits=hits.value)
+
+
+def occupant_picks(seed: int, cell: int, sweep: int, n: int, count: int,
+                   device: int = 0) -> np.ndarray:
+    """The sweep kernel's occupant pick for attempts 0..count-1 of one cell
+    among n (any n in [1, 2^32 - 1]), evaluated on the device."""
+    out = np.zeros(count, dtype=np.uint32)
+    st = load().hsmc_gpu_occupant_picks(device, int(seed), int(cell), int(sweep), int(n),
+                                        int(count), _p(out, C.c_uint32))
+    if st != HSMC_OK:
+        raise HsmcError(st, "occupant_picks")
+    return out
 
 
 def point_descriptors(ctx: "Context", xyz, r: int = 0) -> np.ndarray:

@@ -134,6 +134,15 @@ int hsmc_gpu_set_sweep_counter(hsmc_gpu_ctx *ctx, uint64_t sweep);
  * accepted, rejected for overlap, rejected by the acceptance rule. */
 int hsmc_gpu_npt_sweeps(hsmc_gpu_ctx *ctx, int n_sweeps, const double *press,
                         const double *dv_max, uint64_t *vol_counts);
+/* The sweep's occupant pick (rng_get_int, rng.c:34-36: GSL's scale-and-reject
+ * uniform_int, here on the Philox stream of replica seed `seed`), evaluated
+ * by the same device function the sweep kernel calls: out[t] = pick of
+ * attempt t (t < count <= 65536) of cell `cell` in sweep `sweep` among n
+ * occupants.  n may be any value in [1, 2^32 - 1], so a caller can check the
+ * rule's range, uniformity and redraw path, which a cell of at most 16
+ * occupants almost never takes.  No context is needed. */
+int hsmc_gpu_occupant_picks(int device, uint64_t seed, uint32_t cell, uint64_t sweep, uint32_t n,
+                            int32_t count, uint32_t *out);
 int hsmc_gpu_set_tuning(hsmc_gpu_ctx *ctx, const hsmc_gpu_tuning *t);
 int hsmc_gpu_get_tuning(const hsmc_gpu_ctx *ctx, hsmc_gpu_tuning *t);
 /* Time the candidate launch shapes on this context's own replicas (their

@@ -708,6 +708,24 @@ void orc_cb_sweep_randoms(const orc_cb *c, uint64_t sweep, double shift[3], int 
   }
 }
 
+/* Occupant pick of attempt t of cell cid in sweep `sweep` among n occupants:
+ * rng_get_int's rule (rng.c:34-36, GSL uniform_int: scale = (2^32-1)/n,
+ * k = draw/scale, redraw while k >= n) on the Philox stream, the first draw
+ * from counter (cid, t, sweep) and redraw r from (cid, t | r << 16, sweep). */
+uint32_t orc_cb_pick(const uint32_t key[2], uint32_t cid, uint32_t t, uint64_t sweep, uint32_t n) {
+  const uint32_t scale = 0xFFFFFFFFu / n;
+  uint32_t ctr[4] = {cid, t, (uint32_t)sweep, (uint32_t)(sweep >> 32)};
+  uint32_t w[4];
+  orc_philox4x32_10(ctr, key, w);
+  uint32_t k = w[0] / scale;
+  for (uint32_t r = 1; k >= n; r++) {
+    ctr[1] = t | (r << 16);
+    orc_philox4x32_10(ctr, key, w);
+    k = w[0] / scale;
+  }
+  return k;
+}
+
 /* cell index on the shifted grid: (int)((x - shift, wrapped) * (1/side)) */
 static inline int cb_axis_cell(double x, double shift, double L, double inv_side, int g) {
   double t = x - shift;
@@ -766,14 +784,7 @@ int orc_cb_sweeps(orc_cb *c, int n_sweeps) {
               uint32_t ctr[4] = {(uint32_t)cid, (uint32_t)t, (uint32_t)s, (uint32_t)(s >> 32)};
               uint32_t w[4];
               orc_philox4x32_10(ctr, c->key, w);
-              const uint32_t scale = 0xFFFFFFFFu / (uint32_t)n;
-              uint32_t k = w[0] / scale;
-              for (uint32_t r = 1; k >= (uint32_t)n; r++) {   /* GSL-style rejection */
-                uint32_t ctr2[4] = {(uint32_t)cid, (uint32_t)t | (r << 16), (uint32_t)s, (uint32_t)(s >> 32)};
-                uint32_t w2[4];
-                orc_philox4x32_10(ctr2, c->key, w2);
-                k = w2[0] / scale;
-              }
+              const uint32_t k = orc_cb_pick(c->key, (uint32_t)cid, (uint32_t)t, s, (uint32_t)n);
               const int pid = order[start[cid] + (int)k];
               double nx = c->x[pid] + ((double)w[1] / (double)4294967295.0 - 0.5) * c->dr_max;
               double ny = c->y[pid] + ((double)w[2] / (double)4294967295.0 - 0.5) * c->dr_max;

@@ -150,6 +150,9 @@ void    orc_cb_get_config(const orc_cb *c, double *conf4);
 void    orc_cb_get_counters(const orc_cb *c, uint64_t out[4]);
 /* per-sweep random quantities, exposed so tests can pin them */
 void    orc_cb_sweep_randoms(const orc_cb *c, uint64_t sweep, double shift[3], int perm[8]);
+/* occupant pick (rng_get_int rule, rng.c:34-36) on the checkerboard's Philox
+ * stream; n in [1, 2^32-1] */
+uint32_t orc_cb_pick(const uint32_t key[2], uint32_t cid, uint32_t t, uint64_t sweep, uint32_t n);
 
 #ifdef __cplusplus
 }

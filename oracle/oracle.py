@@ -73,6 +73,7 @@ def lib() -> C.CDLL:
             "orc_cb_get_config": (None, [vp, P(f64)]),
             "orc_cb_get_counters": (None, [vp, P(u64)]),
             "orc_cb_sweep_randoms": (None, [vp, u64, P(f64), P(i32)]),
+            "orc_cb_pick": (C.c_uint32, [P(C.c_uint32), C.c_uint32, C.c_uint32, u64, C.c_uint32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -113,6 +114,12 @@ def philox(ctr, key) -> tuple:
     out = np.zeros(4, dtype=np.uint32)
     lib().orc_philox4x32_10(_p(c, C.c_uint32), _p(k, C.c_uint32), _p(out, C.c_uint32))
     return tuple(int(v) for v in out)
+
+
+def cb_pick(seed: int, cell: int, t: int, sweep: int, n: int) -> int:
+    """The checkerboard sweep's occupant pick (orc_cb_pick)."""
+    k = np.array([seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF], dtype=np.uint32)
+    return int(lib().orc_cb_pick(_p(k, C.c_uint32), cell, t, sweep, n))
 
 
 class OracleError(RuntimeError):

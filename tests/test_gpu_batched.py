@@ -178,3 +178,46 @@ def test_widom_mu_matches_published_table():
         assert abs(mu - ref) < 3.0 * se + 0.03, got
     ctx.close()
 
+
+
+def test_full_c3_batch_properties_and_oracle_replicas():
+    """The whole batched workload: 256 replicas, N = 4n^3 with n uniform in
+    3..15 and rho uniform in [0.30, 0.93], each with its own seed, in one
+    context.  After a few sweeps every replica keeps its hard core (no pair
+    closer than one diameter), every replica made N attempts per sweep, and
+    three replicas (the largest, N = 13500 when drawn, a mid-size one and a
+    dense one) equal the oracle's restatement of the same sweeps bit for bit."""
+    rng = np.random.default_rng(2024)
+    reps = []
+    for r in range(256):
+        n = int(rng.integers(3, 16))
+        rho = float(rng.uniform(0.30, 0.93))
+        conf, box = gpu.lattice(2, n, rho)
+        reps.append(dict(n=len(conf), box=box, dr_max=0.1, seed=1000 + r, conf=conf, rho=rho))
+    sizes = np.array([rep["n"] for rep in reps])
+    assert sizes.min() == 108 and sizes.max() == 13500
+    ctx = gpu.Context(reps)
+    for r, rep in enumerate(reps):
+        ctx.upload(rep["conf"], r)
+    ctx.autotune(8)
+    ctx.reset_counters()
+    sweeps = 6
+    ctx.sweeps(sweeps)
+    assert np.all(ctx.any_overlap_scaled_batch(1.0) == 1)
+    acc = 0
+    for r, rep in enumerate(reps):
+        c = ctx.counters(r)
+        assert c["attempts"] == rep["n"] * sweeps, r
+        assert c["accepted"] + c["left_cell"] <= c["attempts"]
+        acc += c["accepted"]
+    assert acc > 0.2 * sweeps * sizes.sum()
+    picks = [int(np.argmax(sizes)), int(np.argsort(sizes)[128]),
+             int(np.argmax([rep["rho"] for rep in reps]))]
+    for r in picks:
+        rep = reps[r]
+        cb = O.Checkerboard(rep["conf"], rep["box"], ctx.grid(r), 0.1, rep["seed"])
+        cb.sweeps(sweeps)
+        assert np.array_equal(ctx.download(r), cb.config()), (r, rep["n"], rep["rho"])
+        c = ctx.counters(r)
+        assert (c["attempts"], c["accepted"], c["left_cell"]) == \
+            tuple(int(v) for v in cb.counters()[:3]), r

@@ -144,3 +144,61 @@ def test_voxel_scores_match_point_path(n, rho):
     assert np.quantile(diff, 0.99) <= 1e-4 * scale, np.quantile(diff, 0.99)
     assert diff.max() <= 0.05 * scale, diff.max()
     ctx.close()
+
+
+def test_two_networks_share_a_context():
+    """Two networks on one context alternate learned Widom calls with
+    voxel_scores calls, all of which write the context's shared score and
+    cumulative-mass buffers: each network's estimate must equal the one it
+    gave from a cold cache (same seed and sample, bit for bit), never one
+    computed from the other network's scores."""
+    ctx, L = _ctx_with(8, 0.6, 30)
+    a, b = gpu.Mlp(ctx, seed=5), gpu.Mlp(ctx, seed=6)
+    b.train(0.25, 3, 0, 50000, epochs=1, batch=4096, lr=2e-3, beta=10.0)
+    cold_a = a.widom(0.25, 0.9, 21, 1, 100000)["estimate"]
+    cold_b = b.widom(0.25, 0.9, 21, 1, 100000)["estimate"]
+    assert cold_a != cold_b   # different networks, different proposals
+    for _ in range(2):
+        assert a.widom(0.25, 0.9, 21, 1, 100000)["estimate"] == cold_a
+        assert b.widom(0.25, 0.9, 21, 1, 100000)["estimate"] == cold_b
+        b.voxel_scores(0.25)
+        assert a.widom(0.25, 0.9, 21, 1, 100000)["estimate"] == cold_a
+        a.voxel_scores(0.25)
+        assert b.widom(0.25, 0.9, 21, 1, 100000)["estimate"] == cold_b
+    # a different mixing weight rebuilds the cumulative mass, and returning
+    # to the first weight still gives the cold estimate
+    a.widom(0.25, 0.5, 21, 1, 100000)
+    assert b.widom(0.25, 0.9, 21, 1, 100000)["estimate"] == cold_b
+    assert a.widom(0.25, 0.9, 21, 1, 100000)["estimate"] == cold_a
+
+
+def test_learned_widom_at_full_size():
+    """The learned proposal at its own size: N = 108000 (fcc 4*30^3) at
+    rho = 0.80, 206^3 voxels.  On one configuration the learned estimate of
+    the insertion probability agrees with uniform insertion within 3
+    combined standard errors at the same number of insertions, and the tiled
+    voxel scores equal the network applied to per-point FP64 descriptors at
+    the voxel centres on a sample of voxels."""
+    ctx, L = _ctx_with(30, 0.80, 400, seed=3)
+    n_per, samples = 20_000_000, 16
+    pu, su = _uniform_p(ctx, n_per, samples, 77)
+    net = gpu.Mlp(ctx, seed=12)
+    net.train(0.25, 55, 0, 1_000_000, epochs=4, batch=4096, lr=2e-3, beta=20.0)
+    est = [net.widom(0.25, 0.9, 91, s, n_per)["estimate"] for s in range(samples)]
+    pl, sl = float(np.mean(est)), float(np.std(est, ddof=1) / math.sqrt(samples))
+    print(f"N=108000 rho=0.8: P_ins uniform {pu:.5e}+-{su:.1e} learned {pl:.5e}+-{sl:.1e}")
+    assert pu > 0 and sl > 0
+    assert abs(pl - pu) < 3 * math.hypot(su, sl), (pl, pu, su, sl)
+    scores = net.voxel_scores(0.25)
+    m = int(math.ceil(L / 0.25))
+    assert m == 206 and len(scores) == m ** 3
+    v = np.random.default_rng(8).choice(m ** 3, size=20000, replace=False)
+    side = L / m
+    ix, iy, iz = v // (m * m), (v // m) % m, v % m
+    centres = np.stack([(ix + 0.5) * side, (iy + 0.5) * side, (iz + 0.5) * side], axis=1)
+    ref = net.forward(gpu.point_descriptors(ctx, centres))
+    diff = np.abs(scores[v] - ref)
+    scale = max(1.0, float(np.max(np.abs(ref))))
+    assert np.quantile(diff, 0.99) <= 1e-4 * scale, np.quantile(diff, 0.99)
+    assert diff.max() <= 0.05 * scale, diff.max()
+    ctx.close()

@@ -167,3 +167,83 @@ def test_npt_packing_fraction_matches_published_table():
         assert abs(m.mean() - ref) < tol, (p, m.mean(), ref, se)
     ctx.close()
 
+
+
+def test_npt_sweeps_ideal_gas_mean_volume():
+    """The batched volume move's acceptance exp(-beta P dV + (N+1) ln(V'/V))
+    (moves.c:117-121, in hsmc_gpu_npt_sweeps) with a proposal symmetric in
+    ln V samples V ~ Gamma(N+1, beta P) whenever no pair can overlap, so
+    <V> = (N+1)/beta P.  16 replicas of 8 spheres about 11 diameters apart
+    (simple cubic, tiny displacements: they never meet) must give that
+    mean; a slip to N or N+2 moves it by 11%, many standard errors away."""
+    N, bp, R = 8, 8e-4, 16
+    rho = N * bp / (N + 1)
+    reps, confs = [], []
+    for r in range(R):
+        conf, box = gpu.lattice(1, 2, rho)
+        reps.append(dict(n=len(conf), box=box, dr_max=0.05, seed=500 + r))
+        confs.append(conf)
+    ctx = gpu.Context(reps)
+    for r, c in enumerate(confs):
+        ctx.upload(c, r)
+    ctx.npt_sweeps(50, bp, 1.0)   # forget the start
+    vols = []
+    counts = np.zeros((R, 4), dtype=np.uint64)
+    for _ in range(240):
+        counts += ctx.npt_sweeps(4, bp, 1.0)
+        vols.append([float(np.prod(ctx.box(r))) for r in range(R)])
+    vols = np.array(vols)                       # samples x replicas
+    means = vols.mean(axis=0)
+    mean, err = means.mean(), means.std(ddof=1) / math.sqrt(R)
+    want = (N + 1) / bp
+    print(f"<V> = {mean:.1f} +- {err:.1f} (closed form {want:.1f})")
+    assert abs(mean - want) < 4 * err, (mean, want, err)
+    for slip in (N / bp, (N + 2) / bp):
+        assert abs(mean - slip) > 8 * err, (mean, slip, err)
+    assert np.all(counts[:, 0] == counts[:, 1] + counts[:, 2] + counts[:, 3])
+    assert counts[:, 2].sum() == 0              # no overlap ever
+    assert counts[:, 1].sum() > 0.3 * counts[:, 0].sum()
+
+
+def test_c5_full_size_npt_workload():
+    """The NpT workload at its own size: N = 256000 (fcc 4*40^3) at the
+    Carnahan-Starling densities of beta P = 8 and 11, two replicas each in
+    one batched context, ln V step 0.001.  After NpT sweeps: no pair closer
+    than one diameter in any replica; N attempts per replica per sweep; every
+    proposed volume move is accepted, rejected for an overlap or rejected by
+    the rule; and on the N = 256000 configuration the scaled-overlap
+    decisions (vol_move's scan, presst's flags) equal the oracle's."""
+    specs = [(40, 0.8577, 11, 0.1), (40, 0.8577, 12, 0.1), (40, 0.9285, 13, 0.1),
+             (40, 0.9285, 14, 0.1)]
+    press = np.array([8.0, 8.0, 11.0, 11.0])
+    ctx = make(specs)
+    boxes0 = [ctx.box(r) for r in range(ctx.R)]
+    ctx.reset_counters()
+    sweeps = 30
+    vc = ctx.npt_sweeps(sweeps, press, 0.001)
+    assert np.all(vc[:, 0] == vc[:, 1] + vc[:, 2] + vc[:, 3])
+    assert vc[:, 0].sum() > 0
+    for r in range(ctx.R):
+        c = ctx.counters(r)
+        assert c["attempts"] == 256000 * sweeps, r
+        assert 0 < c["accepted"] < c["attempts"]
+    assert np.all(ctx.any_overlap_scaled_batch(1.0) == 1)
+    # oracle comparison on a replica whose box no volume move has changed
+    # (the oracle rebuilds the box from the density, as sim_box_init does)
+    same = [r for r in range(ctx.R) if ctx.box(r) == boxes0[r]]
+    assert same, "every replica accepted a volume move"
+    r = same[-1]
+    rho = specs[r][1]
+    o = O.System(2, 40, rho=rho, neigh_dr=1.0, max_part=12)
+    assert o.box()["lx"] == ctx.box(r)[0]
+    o.set_config(ctx.download(r))
+    sf = np.array([(1.0 - k * 0.0001) ** (1.0 / 3.0) for k in range(1, 21)])
+    ref = o.presst_flags(sf)
+    got = ctx.any_overlap_scaled(sf, r)
+    assert np.array_equal(got, ref)
+    print(f"replica {r} (rho {rho}): presst flags {ref.tolist()}")
+    idx =np.random.default_rng(3).choice(256000, size=3000, replace=False).astype(np.int32)
+    for s in (0.9999, 0.9995):
+        dec = ctx.overlap_scaled(idx, np.full(len(idx), s), r)
+        want = np.array([o.check_overlap(int(i), s) for i in idx], dtype=np.uint8)
+        assert np.array_equal(dec, want), s

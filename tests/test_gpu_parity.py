@@ -357,3 +357,39 @@ def test_wide_cell_single_replica_matches_oracle(tuning):
     c = ctx.counters()
     assert (c["attempts"], c["accepted"], c["left_cell"], c["sweeps"]) == \
         tuple(int(v) for v in cb.counters())
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 16, 2**30 + 1, 0x80000001, 0xC0000000, 0xFFFFFFFF])
+def test_occupant_picks_match_oracle_and_the_rule(n):
+    """The sweep kernel's occupant pick (rng_get_int's scale-and-reject rule,
+    rng.c:34-36, on the Philox stream) evaluated on the device by the same
+    function the phase kernel calls: bit-exact against the oracle's pick,
+    always in [0, n), and uniform.  For n > 2^31 the scale is 1 and a draw
+    of n or more is redrawn (half the draws at n = 2^31 + 1, a quarter at
+    3 * 2^30): the redraw path runs thousands of times here, while a sweep
+    cell of at most 16 occupants takes it with probability below 2^-27."""
+    count = 65536
+    seed, cell, sweep = 0x123456789ABCDEF, 4321, 77
+    got = gpu.occupant_picks(seed, cell, sweep, n, count)
+    assert got.max() < n
+    ts = np.arange(0, count, 97)
+    ref = np.array([O.cb_pick(seed, cell, int(t), sweep, n) for t in ts], dtype=np.uint32)
+    assert np.array_equal(got[ts], ref)
+    # first draws of each attempt (Philox counter (cell, t, sweep)), from the
+    # oracle's KAT-pinned Philox
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    first = np.array([O.philox([cell, int(t), sweep, 0], key)[0] for t in ts], dtype=np.uint64)
+    scale = 0xFFFFFFFF // n
+    redrawn = (first // scale) >= n
+    tail = (2**32 - n * scale) / 2**32   # share of draws past n whole intervals
+    assert abs(redrawn.mean() - tail) < 0.05
+    if tail > 0.2:
+        assert redrawn.sum() > 100
+    # a kept first draw is scaled down, not reduced modulo n
+    assert np.array_equal(got[ts][~redrawn].astype(np.uint64), first[~redrawn] // scale)
+    # uniformity: 8 equal bins of [0, n) (or the n values themselves)
+    bins = min(n, 8)
+    h = np.bincount((got.astype(np.float64) * bins / n).astype(np.int64), minlength=bins)
+    e = count / bins
+    chi2 = float(np.sum((h - e) ** 2 / e))
+    assert chi2 < (bins - 1) + 6 * math.sqrt(2 * max(bins - 1, 1)) + 10, (n, h)

@@ -24,21 +24,107 @@ def test_mt19937_known_answer():
     assert m.next() == 4123659995
 
 
-def test_mt_double_and_int_semantics():
-    # rng_get_double = draw / (2^32 - 1) in [0, 1] (rng.c:29-31);
-    # rng_get_int = GSL scale-and-reject (rng.c:34-36)
-    a, b = O.MT(7), O.MT(7)
-    for _ in range(1000):
-        u = a.next()
-        assert b.double() == u / 4294967295.0
-    a, b = O.MT(9), O.MT(9)
-    for n in (1, 2, 3, 7, 500, 32000, 2**31 - 1):
+def _numpy_mt_stream(seed, n):
+    """Raw 32-bit outputs of numpy's own MT19937 (legacy RandomState, seeded
+    by init_genrand(seed) as gsl_rng_mt19937 seeds it): an implementation
+    independent of the oracle's."""
+    return np.random.RandomState(seed).randint(0, 2**32, size=n, dtype=np.uint64)
+
+
+def test_mt_stream_matches_an_independent_mt19937():
+    for seed in (1, 7, 4357, 123456789):
+        ref = _numpy_mt_stream(seed, 3000)
+        m = O.MT(seed)
+        assert np.array_equal(np.array([m.next() for _ in range(3000)], dtype=np.uint64), ref)
+
+
+def test_mt_double_is_draw_over_max():
+    # rng_get_double = gsl_rng_get / gsl_rng_max (rng.c:29-31), max = 2^32 - 1
+    # for mt19937: the closed interval [0, 1]
+    ref = _numpy_mt_stream(7, 2000).astype(np.float64) / 4294967295.0
+    m = O.MT(7)
+    assert np.array_equal(np.array([m.double() for _ in range(2000)]), ref)
+
+
+def test_mt_int_discards_the_tail_beyond_whole_intervals():
+    """rng_get_int is gsl_rng_uniform_int (rng.c:34-36; its argument is an
+    int, so n < 2^31), documented to reach [0, n) "by scaling down and/or
+    discarding samples": a draw is divided by the integer scale
+    floor((2^32-1)/n), and a quotient of n or more (the tail past n whole
+    intervals) is discarded and drawn again.  With n = 2^30 + 1 (scale 3) a
+    quarter of all draws fall in that tail, with n = 0x55555556 (scale 2) a
+    third, so the outputs must be exactly the in-range quotients of the
+    independent raw stream, in order.  A modulo reduction, a scale of 2^32/n,
+    a multiply-shift map or a tail folded back into range all fail."""
+    for n, seed in ((2**30 + 1, 11), (0x55555556, 12)):
         scale = 0xFFFFFFFF // n
-        while True:
-            k = a.next() // scale
-            if k < n:
-                break
-        assert b.int(n) == k
+        raw = _numpy_mt_stream(seed, 4000)
+        q = raw // scale
+        kept = q[q < n]
+        assert 0.2 < 1.0 - len(kept) / len(raw) < 0.4   # the discard path is taken often
+        m = O.MT(seed)
+        got = np.array([m.int(n) for _ in range(len(kept))], dtype=np.uint64)
+        assert np.array_equal(got, kept)
+        # each discarded draw consumed exactly one output: the stream now
+        # stands right after the last kept draw
+        last = int(np.nonzero(q < n)[0][-1])
+        assert m.next() == int(_numpy_mt_stream(seed, last + 2)[last + 1])
+
+
+def test_mt_int_scales_down_into_equal_intervals():
+    """For small n the rule scales: the output is a nondecreasing function of
+    the raw draw, each value taking one interval of equal width (up to one
+    unit), and all n values appear equally often (chi-square).  A modulo or
+    parity rule is not monotone in the draw."""
+    for n, seed in ((2, 21), (7, 22), (500, 23)):
+        raw = _numpy_mt_stream(seed, 60000)
+        m = O.MT(seed)
+        k = np.array([m.int(n) for _ in range(60000)], dtype=np.int64)
+        assert k.min() >= 0 and k.max() < n
+        # no draw of these streams falls in the discarded tail (at most n-1
+        # values out of 2^32), so outputs and draws stay aligned
+        order = np.argsort(raw, kind="stable")
+        assert np.all(np.diff(k[order]) >= 0)
+        lo = np.array([raw[k == v].min() for v in range(n) if np.any(k == v)], dtype=np.float64)
+        width = 2.0**32 / n
+        if n <= 7:
+            assert np.all(np.abs(lo - np.arange(n) * width) < 2e-3 * width)
+        counts = np.bincount(k, minlength=n)
+        e = len(k) / n
+        chi2 = float(np.sum((counts - e) ** 2 / e))
+        assert chi2 < (n - 1) + 6 * math.sqrt(2 * (n - 1)) + 10, (n, chi2)
+
+
+def test_vol_move_ideal_gas_mean_volume():
+    """vol_move's acceptance exp(-beta P dV + (N+1) ln(V'/V)) (moves.c:117-121)
+    with a proposal symmetric in ln V has the stationary density
+    V^(N+1) exp(-beta P V) in ln V, i.e. V ~ Gamma(N+1, beta P) when no pair
+    ever overlaps: <V> = (N+1)/beta P.  Eight spheres about 11 diameters
+    apart that barely move never overlap, so the oracle's serial NpT chain
+    must give that mean; a slip to N or N+2 moves it by 11%, far outside the
+    blocking error of the run."""
+    from hsmc_amd import analysis as A
+    N, bp = 8, 8e-4
+    # 3 cells per side (the reference keeps its initial cell count in NpT,
+    # so the box may not shrink below 3 diameters; Gamma(9) never gets there)
+    o = O.System(1, 2, rho=N * bp / (N + 1), neigh_dr=7.0, max_part=10, seed=5)
+    assert o.ncells() == 3
+    o.set_params(dr_max=0.05, dv_max=1.0, press=bp)
+    o.sweep_npt(500)   # forget the start
+    vols = np.empty(24000)
+    for k in range(len(vols)):
+        o.sweep_npt(1)
+        vols[k] = o.box()["vol"]
+    mean = vols.mean()
+    err = A.blocking_error(vols)
+    assert err > 0
+    want = (N + 1) / bp
+    assert abs(mean - want) < 4 * err, (mean, want, err)
+    for slip in (N / bp, (N + 2) / bp):
+        assert abs(mean - slip) > 8 * err, (mean, slip, err)
+    # the volume chain moved: several hundred accepted moves per thousand
+    c = o.counters()
+    assert c[4] > 0.3 * c[3]
 
 
 def test_philox_known_answers():
