@@ -1,9 +1,14 @@
 """Headline benchmark: MC move attempts/s of the checkerboard sweep at
-N = 32000, rho = 0.85 (fcc 4*20^3, dr_max = 0.1), on one MI355X.
+N = 32000, rho = 0.85 (fcc 4*20^3, dr_max = 0.1), on one MI355X, and the
+GPU seconds to sigma(beta mu_ex) = 0.01 at rho = 0.90 (N = 108000), learned
+against uniform Widom insertion.
 
-A "step" is one sweep (N attempted moves).  Inputs are resident in HBM when
-the timed region starts.  Prints one JSON line: metric, value, unit, timing,
-the roofline of the sublattice-phase kernel and the 1-core CPU baseline.
+A "step" is one sweep (N attempted moves).  The system is melted from the
+fcc lattice by a stated number of sweeps before the autotuner and the
+warm-up run, so the timed sweeps see the equilibrium fluid.  Inputs are
+resident in HBM when the timed region starts.  Prints one JSON line: metric,
+value, unit, timing, the roofline of the sublattice-phase kernel, the Widom
+comparison (a `widom` object) and the 1-core CPU baseline.
 """
 from __future__ import annotations
 
@@ -22,6 +27,9 @@ HBM_PEAK_GBS = 8000.0
 # at N = 32000 on a 32^3 grid 2,054,144 B / 32000 = 64.19 B per attempt
 N_PART = 32000
 GRID_CELLS = 32 ** 3
+# sweeps that melt the fcc start before tuning and timing (the fluid's
+# acceptance, about 0.53 at dr_max 0.1, is reached well within them)
+SETUP_SWEEPS = 5000
 BYTES_PER_SWEEP = N_PART * 48 + N_PART * 8 + GRID_CELLS * 8
 BYTES_PER_ATTEMPT = BYTES_PER_SWEEP / N_PART
 
@@ -93,15 +101,40 @@ def observables(ctx, no_cpu: bool) -> dict:
     return out
 
 
-def traffic_from_profile():
+def traffic_from_profile(tuning: dict):
+    """HBM bytes per sublattice-phase launch at this tuning, from the
+    calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE runs in profiles/ (None when
+    that tuning has not been profiled)."""
     path = os.path.join(ROOT, "profiles", "sweep_phase_n32000_pmc.json")
     if not os.path.exists(path):
         return None
+    key = f"{tuning['lanes_per_cell']},{tuning['cells_per_block']},{tuning['sweeps_per_graph']}"
     try:
         with open(path) as f:
-            return float(json.load(f)["hbm_bytes_per_launch"])
-    except Exception:
+            entry = json.load(f)["by_tuning"].get(key)
+        return None if entry is None else float(entry["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
         return None
+
+
+def calibrate_copy(mib: int) -> None:
+    """One device-to-device copy of `mib` MiB through the HIP runtime alone
+    (ctypes; neither torch nor the library is loaded): the known-size
+    reference for the rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = C.c_size_t(mib << 20)
+    src, dst = C.c_void_p(), C.c_void_p()
+    for name, ptr in (("src", src), ("dst", dst)):
+        if hip.hipMalloc(C.byref(ptr), nbytes) != 0:
+            raise SystemExit(f"hipMalloc of the {name} buffer failed")
+    if hip.hipMemset(src, 1, nbytes) != 0 or hip.hipDeviceSynchronize() != 0:
+        raise SystemExit("hipMemset failed")
+    if hip.hipMemcpy(dst, src, nbytes, 3) != 0 or hip.hipDeviceSynchronize() != 0:  # 3: device to device
+        raise SystemExit("hipMemcpy failed")
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(f"[bench] copied {mib} MiB device to device", file=sys.stderr)
 
 
 def main() -> None:
@@ -127,8 +160,16 @@ def main() -> None:
     ap.add_argument("--no-phase-timing", action="store_true",
                     help="skip the per-launch HIP-event timing pass (for profiler runs)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
-                    help="only run one device-to-device copy of this many MiB (torch) and exit: "
-                         "the known-size reference for rocprofv3 FETCH_SIZE/WRITE_SIZE counters")
+                    help="only run one device-to-device copy of this many MiB (HIP runtime, no "
+                         "torch) and exit: the known-size reference for rocprofv3 "
+                         "FETCH_SIZE/WRITE_SIZE counters")
+    ap.add_argument("--no-widom", action="store_true",
+                    help="leave the Widom comparison out of the headline line")
+    ap.add_argument("--widom-configs", type=int, default=128,
+                    help="configurations sampled along the Widom chain")
+    ap.add_argument("--widom-gap", type=int, default=50, help="sweeps between Widom samples")
+    ap.add_argument("--profile-widom", action="store_true",
+                    help="only run the Widom kernels on the rho = 0.90 system (for rocprofv3)")
     args = ap.parse_args()
     defaults = {"n32000": (20000, 1000), "batched": (2000, 200), "widom": (0, 0), "npt": (20, 5),
                 "widom_npt": (2000, 20000)}
@@ -137,15 +178,7 @@ def main() -> None:
     if args.warmup is None:
         args.warmup = defaults[args.workload][1]
     if args.calibrate_copy_mib:
-        import torch
-        n = args.calibrate_copy_mib * (1 << 20) // 4
-        a = torch.ones(n, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        print(f"[bench] copied {args.calibrate_copy_mib} MiB", file=sys.stderr)
-        return
+        return calibrate_copy(args.calibrate_copy_mib)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(independent replicas shard across GPUs with no exchange)", file=sys.stderr)
@@ -159,18 +192,22 @@ def main() -> None:
         return npt(args)
     if args.workload == "widom_npt":
         return widom_npt(args)
+    if args.profile_widom:
+        return profile_widom(args)
 
     from hsmc_amd import gpu
     conf, box = gpu.lattice(2, 20, 0.85)
     ctx = gpu.Context([dict(n=len(conf), box=box, dr_max=0.1, seed=1)])
     ctx.upload(conf)
     print(f"[bench] grid {ctx.grid()}  N={len(conf)}", file=sys.stderr, flush=True)
+    ctx.sweeps(SETUP_SWEEPS)   # melt the lattice; the autotuner restores this state
+    ctx.sync()
     if args.tuning:
         ctx.set_tuning(*(int(v) for v in args.tuning.split(",")))
         tuning = ctx.get_tuning()
     else:
         tuning = ctx.get_tuning() if args.no_autotune else ctx.autotune(512)
-    print(f"[bench] tuning {tuning}", file=sys.stderr, flush=True)
+    print(f"[bench] melted by {SETUP_SWEEPS} sweeps; tuning {tuning}", file=sys.stderr, flush=True)
 
     ctx.sweeps(args.warmup)
     ctx.sync()
@@ -205,18 +242,24 @@ def main() -> None:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (fcc 4*20^3 lattice start, Philox proposals)",
+        "data": "synthetic (fcc 4*20^3 lattice start melted by setup_sweeps, Philox proposals)",
         "config": {"workload": "NVT hard spheres N=32000 (fcc 4*20^3) rho=0.85 dr_max=0.1, "
                                "checkerboard sweep on a 32^3 grid",
                    "n_part": N_PART, "rho": 0.85, "dr_max": 0.1, "grid": list(ctx.grid()),
-                   "tuning": tuning, "acceptance": c["accepted"] / c["attempts"]},
+                   "setup_sweeps": SETUP_SWEEPS, "tuning": tuning,
+                   "timed_launches": "graph replays only (a graph of exactly the remaining "
+                                     "steps mod sweeps_per_graph is captured before timing)",
+                   "acceptance": c["accepted"] / c["attempts"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(tuning),
                      "kernel": "k_phase (sublattice phase)", "avg_launch_ms": avg_ms,
                      "launches_timed": launches,
                      "algorithmic_bytes_per_launch": per_launch_bytes},
     }
     line["observables"] = observables(ctx, args.no_cpu)
+    ctx.close()
+    if not args.no_widom:
+        line["widom"] = widom_headline(args)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line))
@@ -334,129 +377,164 @@ def _timed(fn):
     return out, time.perf_counter() - t0
 
 
-def _compare_widom(args, systems, n_uniform, n_learned, label):
-    """GPU seconds of insertion work until sigma(beta mu_ex) reaches the
-    target, plain uniform insertions against the learned proposal, on the
-    same K configurations (one context each; systems = [(conf, box), ...]).
-    The network is trained once on 10^6 uniform insertions of the first
-    configuration (its time is reported apart; the learned seconds include
-    scoring every configuration's voxels)."""
+def _widom_chain(ctx, advance, target, S, gap, n_u, n_l, seed=0, train_points=1_000_000):
+    """Uniform against learned Widom along one running chain of `ctx`
+    (replica 0).  The network is trained on the current configuration first
+    (timed, and charged to the learned method).  Then S samples, each `gap`
+    chain steps after the last (advance(gap); reported apart and charged to
+    neither method, since both use the same configurations): plain uniform
+    insertion with n_u points, and the learned proposal with n_l points,
+    whose call scores the new configuration's voxels and rebuilds its
+    cumulative mass (charged).  sigma(beta mu_ex) comes from blocking over
+    the per-configuration estimates (hsmc_stat.py:6-50), so configurational
+    variance is sampled; GPU seconds to the target sigma follow the
+    1/samples law from the measured seconds."""
     import math
 
     import numpy as np
 
+    from hsmc_amd import analysis as A
     from hsmc_amd import gpu
-    K = len(systems)
     voxel, mix = 0.25, 0.9
-    ctxs = []
-    for k, (conf, box) in enumerate(systems):
-        c = gpu.Context([dict(n=len(conf), box=box, dr_max=0.1, seed=100 + k)])
-        c.upload(conf)
-        ctxs.append(c)
-    net0 = gpu.Mlp(ctxs[0], seed=5)
-    loss, t_train = _timed(lambda: net0.train(voxel, 777, 0, 1_000_000, epochs=10, batch=4096,
-                                               lr=2e-3, beta=20.0))
-    params = net0.get()
-    nets = [net0] + [gpu.Mlp(c, seed=5) for c in ctxs[1:]]
-    for m in nets[1:]:
-        m.set(params)
-    print(f"[bench] {label}: trained, loss {loss:.4f} in {t_train:.3f} s", file=sys.stderr, flush=True)
+    net = gpu.Mlp(ctx, seed=5)
+    ctx.sync()
+    loss, t_train = _timed(lambda: net.train(voxel, 777 + seed, 0, train_points, epochs=10,
+                                             batch=4096, lr=2e-3, beta=20.0))
+    pu, pl, tu, tl = [], [], [], []
+    hits_u = hits_l = 0
+    t_chain = 0.0
+    for k in range(S):
+        _, dt = _timed(lambda: (advance(gap), ctx.sync()))
+        t_chain += dt
+        h, dt = _timed(lambda: ctx.widom_uniform(1234 + seed, k, 0, n_u))
+        pu.append(h / n_u)
+        tu.append(dt)
+        hits_u += h
+        r, dt = _timed(lambda: net.widom(voxel, mix, 4321 + seed, k, n_l))
+        pl.append(r["estimate"])
+        tl.append(dt)
+        hits_l += r["hits"]
+    net.close()
 
-    # at least 16 rounds per configuration, so the variance estimates behind
-    # sigma (and the agreement test) are not themselves too noisy
-    def run(method, n_per, min_rounds=16, max_rounds=4000):
-        est = [[] for _ in range(K)]
-        secs = 0.0
-        n_total = 0
-        rnd = 0
-        while True:
-            for k in range(K):
-                if method == "uniform":
-                    h, dt = _timed(lambda: ctxs[k].widom_uniform(1234 + k, rnd, 0, n_per))
-                    est[k].append(h / n_per)
-                    n_total += n_per
-                else:
-                    r, dt = _timed(lambda: nets[k].widom(voxel, mix, 4321 + k, rnd, n_per))
-                    est[k].append(r["estimate"])
-                    n_total += r["n"]
-                secs += dt
-            rnd += 1
-            if rnd < 2:
-                continue
-            p = np.mean([np.mean(e) for e in est])
-            var = sum(np.var(e, ddof=1) / len(e) for e in est) / K ** 2
-            sig = math.sqrt(var) / p if p > 0 else float("inf")
-            if rnd >= min_rounds and (sig <= args.sigma or rnd >= max_rounds):
-                # seconds_at_target: the time scaled to exactly the target sigma
-                # by the 1/n law (a run that stops past the target is charged
-                # for its overshoot in `seconds`)
-                return dict(beta_mu=-math.log(p), sigma=float(sig), p_insert=float(p), seconds=secs,
-                            seconds_at_target=secs * (float(sig) / args.sigma) ** 2,
-                            insertions=n_total, rounds=rnd,
-                            insertions_per_s=n_total / secs)
+    def summary(p, t, n_per, fixed, hits):
+        p = np.asarray(p, dtype=np.float64)
+        mean = float(p.mean())
+        sig = A.blocking_error(p) / mean if mean > 0 else float("inf")
+        secs = float(sum(t))
+        return dict(beta_mu=-math.log(mean) if mean > 0 else float("inf"), sigma=sig,
+                    p_insert=mean, configurations=S, insertions=n_per * S, hits=int(hits),
+                    seconds_measured=fixed + secs,
+                    seconds_to_target=fixed + secs * (sig / target) ** 2,
+                    ms_per_configuration=1e3 * secs / S, insertions_per_s=n_per * S / secs)
 
-    uni = run("uniform", n_uniform)
-    print(f"[bench] {label} uniform: {uni}", file=sys.stderr, flush=True)
-    lrn = run("learned", n_learned)
-    print(f"[bench] {label} learned: {lrn}", file=sys.stderr, flush=True)
-    for c in ctxs:
-        c.close()
+    uni = summary(pu, tu, n_u, 0.0, hits_u)
+    lrn = summary(pl, tl, n_l, t_train, hits_l)
+    lrn["training_seconds"] = t_train
+    lrn["training_loss"] = loss
     return {
-        "ratio": uni["seconds"] / lrn["seconds"],
-        "ratio_at_target": uni["seconds_at_target"] / lrn["seconds_at_target"],
+        "ratio": uni["seconds_to_target"] / lrn["seconds_to_target"],
         "agreement_sigmas": abs(uni["beta_mu"] - lrn["beta_mu"]) / math.hypot(uni["sigma"], lrn["sigma"]),
-        "uniform": uni, "learned": lrn, "learned_training_seconds": t_train,
+        "target_sigma": target,
+        "uniform": uni, "learned": lrn,
+        "chain": {"gap": gap, "steps": S * gap, "seconds": t_chain},
     }
 
 
-def _widom_density(args, rho, n_uniform, n_learned):
-    """Learned against uniform Widom at density rho, N = 108000 (fcc 4*30^3,
-    melted by 10^4 sweeps), on K = 4 configurations 10^3 sweeps apart."""
-    from hsmc_amd import analysis as A
+def _widom_system(rho, melt, tuning=""):
+    """N = 108000 (fcc 4*30^3) at density rho, melted by `melt` sweeps."""
     from hsmc_amd import gpu
     conf, box = gpu.lattice(2, 30, rho)
-    base = gpu.Context([dict(n=len(conf), box=box, dr_max=0.1, seed=11)])
-    if args.tuning:
-        base.set_tuning(*(int(v) for v in args.tuning.split(",")))
-    base.upload(conf)
-    base.sweeps(10000)   # the superheated fcc crystal melts (mean square displacement grows linearly)
-    systems = []
-    for k in range(4):
-        base.sweeps(1000)
-        systems.append((base.download(), box))
-    base.close()
-    print(f"[bench] widom: 4 configurations of N={len(conf)} at rho={rho}", file=sys.stderr, flush=True)
-    out = _compare_widom(args, systems, n_uniform, n_learned, f"rho {rho}")
-    out["rho"] = rho
-    out["carnahan_starling_beta_mu_ex"] = A.carnahan_starling(rho)["mu_ex"]
+    ctx = gpu.Context([dict(n=len(conf), box=box, dr_max=0.1, seed=11)])
+    ctx.set_tuning(*((int(v) for v in tuning.split(",")) if tuning else (32, 2, 64)))
+    ctx.upload(conf)
+    ctx.sweeps(melt)   # the superheated fcc crystal melts (the mean square displacement grows linearly)
+    ctx.sync()
+    return ctx, box
+
+
+def _widom_kernel_roofline(res, n_part, grid_cells):
+    """Algorithmic bytes of a Widom call (the configuration's positions,
+    cell counts and ids read once: N*24 + C*4 + N*4 per sample, amortised
+    over its insertions) over the measured time per call, against HBM peak."""
+    algo = n_part * 28 + grid_cells * 4
+    out = {}
+    for name in ("uniform", "learned"):
+        ms = res[name]["ms_per_configuration"]
+        gbs = algo / (ms / 1e3) / 1e9
+        out[name] = {"bound": "hbm", "algorithmic_bytes_per_call": algo, "achieved": gbs,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "insertions_per_s": res[name]["insertions_per_s"]}
     return out
 
 
+def widom_headline(args) -> dict:
+    """The Widom half of the headline metric: GPU seconds to sigma(beta
+    mu_ex) = 0.01 at rho = 0.90, N = 108000, learned against uniform, along
+    one running NVT chain (10^4 melt sweeps, then --widom-configs samples
+    --widom-gap sweeps apart)."""
+    from hsmc_amd import analysis as A
+    ctx, box = _widom_system(0.90, 10000, args.tuning)
+    print(f"[bench] widom: N=108000 rho=0.90 melted, box {box[0]:.4f}", file=sys.stderr, flush=True)
+    res = _widom_chain(ctx, ctx.sweeps, args.sigma, args.widom_configs, args.widom_gap,
+                       100_000_000, 1_000_000)
+    ctx.close()
+    print(f"[bench] widom: uniform {res['uniform']}", file=sys.stderr, flush=True)
+    print(f"[bench] widom: learned {res['learned']}", file=sys.stderr, flush=True)
+    g = int(box[0] // 1.0)
+    res["workload"] = ("N=108000 (fcc 4*30^3) rho=0.90 melted by 10^4 sweeps; NVT chain, one "
+                       "sample every gap sweeps; uniform 1e8 and learned 1e6 insertions per "
+                       "sample; voxel 0.25 sigma (198^3), mix 0.9, network 12-64-64-1 trained on "
+                       "1e6 uniform insertions (training charged to the learned seconds)")
+    res["carnahan_starling_beta_mu_ex"] = A.carnahan_starling(0.90)["mu_ex"]
+    res["roofline"] = _widom_kernel_roofline(res, 108000, g ** 3)
+    return res
+
+
+def profile_widom(args) -> None:
+    """Only the Widom kernels on the rho = 0.90, N = 108000 system (a few
+    calls each), for rocprofv3 counter runs."""
+    from hsmc_amd import gpu
+    ctx, _ = _widom_system(0.90, 2000, args.tuning)
+    for k in range(3):
+        ctx.widom_uniform(5, k, 0, 100_000_000)
+    net = gpu.Mlp(ctx, seed=5)
+    net.train(0.25, 7, 0, 100_000, epochs=1, batch=4096, lr=2e-3, beta=20.0)
+    for k in range(3):
+        ctx.sweeps(1)   # a new configuration: the learned call scores it
+        net.widom(0.25, 0.9, 9, k, 1_000_000)
+    ctx.sync()
+    net.close()
+    ctx.close()
+    print("[bench] widom kernels done", file=sys.stderr, flush=True)
+
+
 def widom(args) -> None:
-    """Learned against uniform Widom insertion at rho = 0.70, 0.80, 0.90
-    (N = 108000, voxel 0.25 sigma, mix 0.9); the headline value is rho = 0.90.
-    Insertions per call are sized to each density's insertion probability."""
-    sizes = {0.70: (1_000_000, 100_000), 0.80: (10_000_000, 250_000), 0.90: (50_000_000, 500_000)}
-    res = {str(rho): _widom_density(args, rho, *sizes[rho]) for rho in (0.70, 0.80, 0.90)}
+    """Learned against uniform Widom at rho = 0.70, 0.80, 0.90 (N = 108000,
+    voxel 0.25 sigma, mix 0.9), each along its own running NVT chain; the
+    headline value is rho = 0.90.  Insertions per sample are sized to each
+    density's insertion probability."""
+    from hsmc_amd import analysis as A
+    sizes = {0.70: (2_000_000, 200_000), 0.80: (20_000_000, 500_000), 0.90: (100_000_000, 1_000_000)}
+    res = {}
+    for rho in (0.70, 0.80, 0.90):
+        ctx, _ = _widom_system(rho, 10000, args.tuning)
+        out = _widom_chain(ctx, ctx.sweeps, args.sigma, args.widom_configs, args.widom_gap, *sizes[rho])
+        ctx.close()
+        out["carnahan_starling_beta_mu_ex"] = A.carnahan_starling(rho)["mu_ex"]
+        print(f"[bench] widom rho {rho}: ratio {out['ratio']:.2f}, agreement "
+              f"{out['agreement_sigmas']:.2f} sigma", file=sys.stderr, flush=True)
+        res[str(rho)] = out
     top = res["0.9"]
     print(json.dumps({
         "metric": "GPU seconds to sigma(beta mu_ex)=0.01 at rho=0.90, learned vs uniform Widom",
         "value": top["ratio"], "unit": "x less GPU time (uniform / learned)",
         "n_gpus": 1, "higher_is_better": True, "vs_baseline": None, "dtype": "f64 (overlap), bf16 (network)",
-        "data": "synthetic (fcc 4*30^3 melted by 10^4 sweeps; 4 configurations 10^3 sweeps apart per density)",
-        "config": {"workload": "N=108000, rho=0.90, voxel 0.25 sigma (198^3), mix 0.9, "
+        "data": "synthetic (fcc 4*30^3 melted by 10^4 sweeps; one running NVT chain per density)",
+        "config": {"workload": "N=108000, rho in {0.70, 0.80, 0.90}, voxel 0.25 sigma, mix 0.9, "
                                "network 12-64-64-1 trained on 1e6 uniform insertions",
-                   "target_sigma": args.sigma},
-        "ratio_at_target": top["ratio_at_target"],
-        "uniform": top["uniform"], "learned": top["learned"],
-        "learned_training_seconds": top["learned_training_seconds"],
-        "agreement_sigmas": top["agreement_sigmas"],
-        "carnahan_starling_beta_mu_ex": top["carnahan_starling_beta_mu_ex"],
-        "densities": {k: {kk: v[kk] for kk in ("ratio", "ratio_at_target", "agreement_sigmas",
-                                                "carnahan_starling_beta_mu_ex")}
-                      | {"beta_mu_uniform": v["uniform"]["beta_mu"], "beta_mu_learned": v["learned"]["beta_mu"],
-                         "seconds_uniform": v["uniform"]["seconds"], "seconds_learned": v["learned"]["seconds"]}
-                      for k, v in res.items()},
+                   "target_sigma": args.sigma, "configurations": args.widom_configs,
+                   "gap_sweeps": args.widom_gap},
+        "densities": res,
     }))
 
 def _cs_density(p):
@@ -473,84 +551,56 @@ def _cs_density(p):
 
 
 def widom_npt(args) -> None:
-    """Learned against uniform Widom on NpT configurations of N = 256000 at
-    beta P = 5, 8, 11.  Per pressure, a batched context of 4 replicas starts
-    from fcc 4*40^3 at the Carnahan-Starling density of that pressure, melts
-    in NVT sweeps (--warmup; the crystal at rho 0.93 needs about 1.6e4 to
-    start diffusing), then runs NpT sweeps (--steps, ln-volume step 0.001);
-    its 4 final configurations (and boxes) are the Widom systems.  The
-    headline value is beta P = 11.  At N = 256000 an ln-volume step of up to
-    5e-4 compresses hundreds of contact pairs into overlap, so about 0.1% of
-    volume moves are accepted and the density stays near its start."""
+    """Learned against uniform Widom on NpT chains of N = 256000 at beta P =
+    5, 8, 11.  Per pressure one system starts from fcc 4*40^3 at the
+    Carnahan-Starling density of that pressure, melts in NVT sweeps
+    (--warmup; the crystal at rho 0.93 needs about 1.6e4 to start
+    diffusing), runs NpT sweeps (--steps, ln-volume step 0.001), and then
+    the Widom chain continues with NpT sweeps between samples.  At N = 256000
+    an ln-volume step of up to 5e-4 compresses hundreds of contact pairs into
+    overlap, so about 0.1% of volume moves are accepted and the density stays
+    near its start.  The headline value is beta P = 11."""
     import numpy as np
 
     from hsmc_amd import analysis as A
     from hsmc_amd import gpu
-    levels, K, n_cells, dv = [5.0, 8.0, 11.0], 4, 40, 0.001
+    levels, n_cells, dv = [5.0, 8.0, 11.0], 40, 0.001
     melt, settle = args.warmup, args.steps
-    systems, vol, msd = {}, {}, {}
-    for p in levels:
-        conf, box = gpu.lattice(2, n_cells, _cs_density(p))
-        ctx = gpu.Context([dict(n=len(conf), box=box, dr_max=0.1, seed=500 + 10 * int(p) + k)
-                           for k in range(K)])
-        if args.tuning:
-            ctx.set_tuning(*(int(v) for v in args.tuning.split(",")))
-        for k in range(K):
-            ctx.upload(conf, k)
-        ctx.sweeps(melt)
-        vol[p] = ctx.npt_sweeps(settle, [p] * K, dv)
-        if not all(ctx.any_overlap_scaled_batch(1.0) == 1):
-            raise SystemExit("overlap found after the NpT sweeps")
-        # mean square displacement from the lattice, minimum image per axis
-        # (a lower bound once spheres travel beyond L/2); the crystal's
-        # vibrations alone give about 0.01
-        m = []
-        for k in range(K):
-            d = ctx.download(k)[:, 1:] - conf[:, 1:]
-            L = np.array(ctx.box(k))
-            d -= L * np.round(d / L)
-            m.append(float((d * d).sum(1).mean()))
-        msd[p] = float(np.mean(m))
-        systems[p] = [(ctx.download(k), tuple(ctx.box(k))) for k in range(K)]
-        dens = [len(conf) / float(np.prod(b)) for _, b in systems[p]]
-        print(f"[bench] widom_npt: beta P {p}: density {np.mean(dens):.5f} (start {_cs_density(p):.5f}), "
-              f"msd {msd[p]:.3f}, volume moves {int(vol[p][:, 1].sum())} of {int(vol[p][:, 0].sum())} "
-              f"accepted", file=sys.stderr, flush=True)
-        ctx.close()
-    sizes = {5.0: (1_000_000, 100_000), 8.0: (25_000_000, 250_000), 11.0: (200_000_000, 500_000)}
+    sizes = {5.0: (2_000_000, 200_000), 8.0: (25_000_000, 500_000), 11.0: (200_000_000, 1_000_000)}
     res = {}
     for p in levels:
-        out = _compare_widom(args, systems[p], *sizes[p], f"beta P {p}")
-        dens = [len(c) / float(np.prod(b)) for c, b in systems[p]]
-        out["beta_P"] = p
-        out["density"] = float(np.mean(dens))
-        out["volume_moves_accepted"] = f"{int(vol[p][:, 1].sum())} of {int(vol[p][:, 0].sum())}"
-        out["msd_from_lattice"] = msd[p]
-        out["carnahan_starling_density"] = _cs_density(p)
-        out["carnahan_starling_beta_mu_ex"] = A.carnahan_starling(float(np.mean(dens)))["mu_ex"]
+        conf, box = gpu.lattice(2, n_cells, _cs_density(p))
+        ctx = gpu.Context([dict(n=len(conf), box=box, dr_max=0.1, seed=500 + int(p))])
+        ctx.set_tuning(*((int(v) for v in args.tuning.split(",")) if args.tuning else (32, 2, 64)))
+        ctx.upload(conf)
+        ctx.sweeps(melt)
+        vol = ctx.npt_sweeps(settle, [p], dv)
+        advance = lambda g, p=p: ctx.npt_sweeps(g, [p], dv)
+        out = _widom_chain(ctx, advance, args.sigma, args.widom_configs, args.widom_gap, *sizes[p],
+                           seed=int(p))
+        if ctx.any_overlap_scaled([1.0])[0] != 1:
+            raise SystemExit("overlap found after the NpT sweeps")
+        dens = len(conf) / float(np.prod(ctx.box()))
+        ctx.close()
+        out.update(beta_P=p, density=dens, carnahan_starling_density=_cs_density(p),
+                   carnahan_starling_beta_mu_ex=A.carnahan_starling(dens)["mu_ex"],
+                   settle_volume_moves_accepted=f"{int(vol[0, 1])} of {int(vol[0, 0])}")
+        print(f"[bench] widom_npt beta P {p}: density {dens:.5f}, ratio {out['ratio']:.2f}, "
+              f"agreement {out['agreement_sigmas']:.2f} sigma", file=sys.stderr, flush=True)
         res[str(p)] = out
     top = res["11.0"]
     print(json.dumps({
-        "metric": "GPU seconds to sigma(beta mu_ex)=0.01 on NpT configurations at beta P = 11, "
+        "metric": "GPU seconds to sigma(beta mu_ex)=0.01 on NpT chains at beta P = 11, "
                   "learned vs uniform Widom",
         "value": top["ratio"], "unit": "x less GPU time (uniform / learned)",
         "n_gpus": 1, "higher_is_better": True, "vs_baseline": None, "dtype": "f64 (overlap), bf16 (network)",
         "data": f"synthetic (fcc 4*40^3 at the Carnahan-Starling density of each pressure, {melt} NVT "
-                f"sweeps, then {settle} NpT sweeps per replica)",
-        "config": {"workload": "N=256000 NpT configurations, 4 replicas (one batched context) at each "
-                               "beta P in {5, 8, 11}, "
-                               "voxel 0.25 sigma, mix 0.9, network 12-64-64-1 trained on 1e6 uniform "
-                               "insertions per pressure",
-                   "target_sigma": args.sigma, "melt_sweeps": melt, "npt_sweeps": settle},
-        "ratio_at_target": top["ratio_at_target"], "agreement_sigmas": top["agreement_sigmas"],
-        "pressures": {k: {kk: v[kk] for kk in ("ratio", "ratio_at_target", "agreement_sigmas", "density",
-                                                "carnahan_starling_density", "carnahan_starling_beta_mu_ex",
-                                                "learned_training_seconds", "volume_moves_accepted",
-                                                "msd_from_lattice")}
-                      | {"beta_mu_uniform": v["uniform"]["beta_mu"], "beta_mu_learned": v["learned"]["beta_mu"],
-                         "sigma_uniform": v["uniform"]["sigma"], "sigma_learned": v["learned"]["sigma"],
-                         "seconds_uniform": v["uniform"]["seconds"], "seconds_learned": v["learned"]["seconds"]}
-                      for k, v in res.items()},
+                f"sweeps, {settle} NpT sweeps, then NpT sweeps between Widom samples)",
+        "config": {"workload": "N=256000 NpT chains at beta P in {5, 8, 11}, voxel 0.25 sigma, mix 0.9, "
+                               "network 12-64-64-1 trained on 1e6 uniform insertions per pressure",
+                   "target_sigma": args.sigma, "melt_sweeps": melt, "npt_sweeps": settle,
+                   "configurations": args.widom_configs, "gap_sweeps": args.widom_gap},
+        "pressures": res,
     }))
 
 

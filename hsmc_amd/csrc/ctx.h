@@ -107,6 +107,12 @@ struct hsmc_gpu_ctx {
   hipGraphExec_t gexec = nullptr;
   int graph_sweeps = 0;
   hsmc_gpu_tuning graph_tune{0, 0, 0, 0};
+  // a second captured graph for a call's remainder (n mod sweeps_per_graph
+  // sweeps), captured by the timing call outside its timed region
+  hipGraph_t graph_r = nullptr;
+  hipGraphExec_t gexec_r = nullptr;
+  int graph_r_sweeps = 0;
+  hsmc_gpu_tuning graph_r_tune{0, 0, 0, 0};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string last_error;
   // configuration version (bumped by anything that moves particles) and the
@@ -156,6 +162,14 @@ int check_device_errors(hsmc_gpu_ctx *ctx);
 void drop_graph(hsmc_gpu_ctx *ctx);
 // Add the per-cell device counters into the host totals and clear them.
 int fold_counters(hsmc_gpu_ctx *ctx);
+// Workspace bytes the scaled-overlap scan of a replica needs (0 for bad
+// arguments), and the scan itself with every buffer carved from a device
+// workspace of ws_bytes (nothing allocated).
+size_t overlap_scan_workspace_bytes(const hsmc_gpu_ctx *ctx, int replica, const double *sf, int n_sf);
+int overlap_scan_in_workspace(hsmc_gpu_ctx *ctx, int replica, const double *sf, int n_sf,
+                              unsigned char *ws, size_t ws_bytes, uint8_t *no_overlap);
+// Look up the allocation-free scratch array's address on `device` (current).
+int noalloc_scratch_init(int device);
 
 }  // namespace hsmc
 

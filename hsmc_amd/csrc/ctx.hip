@@ -53,6 +53,11 @@ void drop_graph(hsmc_gpu_ctx *ctx) {
   ctx->gexec = nullptr;
   ctx->graph = nullptr;
   ctx->graph_sweeps = 0;
+  if (ctx->gexec_r) (void)hipGraphExecDestroy(ctx->gexec_r);
+  if (ctx->graph_r) (void)hipGraphDestroy(ctx->graph_r);
+  ctx->gexec_r = nullptr;
+  ctx->graph_r = nullptr;
+  ctx->graph_r_sweeps = 0;
 }
 
 int fold_counters(hsmc_gpu_ctx *ctx) {
@@ -351,6 +356,8 @@ int hsmc_gpu_create(hsmc_gpu_ctx **out, int device, int n_replicas,
     return code;
   };
   if (hipSetDevice(device) != hipSuccess) return fail(HSMC_ERR_HIP);
+  // the allocation-free entry point's scratch array, resolved once per device
+  if (hsmc::noalloc_scratch_init(device) != HSMC_OK) return fail(HSMC_ERR_HIP);
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(HSMC_ERR_HIP);
   if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess)

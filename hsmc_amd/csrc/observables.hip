@@ -203,6 +203,108 @@ k_widom_uniform(Grid g, uint32_t k0, uint32_t k1, unsigned long long sample, lon
   if (threadIdx.x == 0 && s) atomicAdd(hits, s);
 }
 
+// Cell-stratified uniform insertion: every cell of the observable grid
+// (side >= 1) receives `per_cell` points uniform inside it.  The cells have
+// equal volume, so hits / (per_cell * C) is an unbiased estimate of the
+// insertion probability, like widom_insertion's uniform points
+// (compute_widom_chem_pot.c:44-80), with a variance no larger.  Point j of
+// cell c comes from Philox counter {c, j, sample_lo, 0x5743xxxx}:
+// coordinate (ic + (u + 0.5) 2^-32) * side per axis.
+//
+// One wave per cell: lanes j < 27 read the neighbour cells' ranges, and the
+// spheres of the 27 cells are staged once in LDS in the order own cell, 6
+// faces, 12 edges, 8 corners (the likeliest overlaps first).  The wave then
+// takes 64 points at a time, one per lane, and walks the list in lockstep
+// (every lane reads entry e together: an LDS broadcast) with the
+// reference's FP64 distance (moves.c:400-431 order, minimum image), leaving
+// the walk as soon as a ballot shows every point overlapping.
+constexpr int kWidomCellsLcap = 256;   // bound of spheres in a 27-cell stencil (host-checked)
+constexpr int kWidomCellsWaves = 4;    // waves per workgroup
+
+// stencil lane order: distance class of offset (dx,dy,dz) is |dx|+|dy|+|dz|
+__device__ __forceinline__ int stencil_by_distance(int j, int &dx, int &dy, int &dz) {
+  // j in [0,27): the 27 offsets sorted by (|dx|+|dy|+|dz|, lexicographic)
+  int k = 0;
+  for (int cls = 0; cls <= 3; ++cls)
+    for (int t = 0; t < 27; ++t) {
+      const int ox = t / 9 - 1, oy = (t / 3) % 3 - 1, oz = t % 3 - 1;
+      if (abs(ox) + abs(oy) + abs(oz) != cls) continue;
+      if (k == j) {
+        dx = ox; dy = oy; dz = oz;
+        return cls;
+      }
+      ++k;
+    }
+  return -1;
+}
+
+__global__ void __launch_bounds__(64 * kWidomCellsWaves)
+k_widom_cells(Grid g, uint32_t k0, uint32_t k1, unsigned long long sample, int per_cell,
+              unsigned long long *__restrict__ hits, unsigned *__restrict__ err) {
+  __shared__ double sx_[kWidomCellsWaves][kWidomCellsLcap];
+  __shared__ double sy_[kWidomCellsWaves][kWidomCellsLcap];
+  __shared__ double sz_[kWidomCellsWaves][kWidomCellsLcap];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double *Lx = sx_[w], *Ly = sy_[w], *Lz = sz_[w];
+  const long C = (long)g.nx * g.ny * g.nz;
+  const uint32_t c3 = 0x57430000u | (uint32_t)((sample >> 32) & 0xFFFFu);
+  unsigned long long h = 0;
+  for (long c = (long)blockIdx.x * kWidomCellsWaves + w; c < C; c += (long)gridDim.x * kWidomCellsWaves) {
+    const int iz = (int)(c % g.nz);
+    const long t2 = c / g.nz;
+    const int iy = (int)(t2 % g.ny), ix = (int)(t2 / g.ny);
+    // neighbour ranges (distinct cells only on axes of fewer than 3 cells)
+    int b = 0, n = 0;
+    if (lane < 27) {
+      int dx = 0, dy = 0, dz = 0;
+      stencil_by_distance(lane, dx, dy, dz);
+      if (stencil_axis_ok(dx, g.nx) && stencil_axis_ok(dy, g.ny) && stencil_axis_ok(dz, g.nz)) {
+        const int nc = (wrap_index(ix + dx, g.nx) * g.ny + wrap_index(iy + dy, g.ny)) * g.nz +
+                       wrap_index(iz + dz, g.nz);
+        b = g.start[nc];
+        n = g.start[nc + 1] - b;
+      }
+    }
+    int incl = n;
+    for (int d = 1; d < 32; d <<= 1) {
+      const int v = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += v;
+    }
+    const int T = __shfl(incl, 26, 64);
+    if (T > kWidomCellsLcap) {  // above the packing bound the host checks: cannot happen
+      if (lane == 0) atomicOr(err, kErrListFull);
+      continue;
+    }
+    for (int q = 0, o = incl - n; q < n; ++q) {
+      Lx[o + q] = g.px[b + q];
+      Ly[o + q] = g.py[b + q];
+      Lz[o + q] = g.pz[b + q];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // 64 points at a time: point base + lane of the cell
+    for (int base = 0; base < per_cell; base += 64) {
+      const int jj = base + lane;
+      const bool mine = jj < per_cell;
+      uint32_t u[4];
+      philox4x32_10((uint32_t)c, (uint32_t)jj, (uint32_t)sample, c3, k0, k1, u);
+      const double qx = ((double)ix + ((double)u[0] + 0.5) * 0x1p-32) * g.sx;
+      const double qy = ((double)iy + ((double)u[1] + 0.5) * 0x1p-32) * g.sy;
+      const double qz = ((double)iz + ((double)u[2] + 0.5) * 0x1p-32) * g.sz;
+      bool ov = false;
+      for (int e = 0; e < T; ++e) {
+        ov |= dist2(qx, qy, qz, Lx[e], Ly[e], Lz[e], g.lx, g.ly, g.lz, g.hx, g.hy, g.hz) < 1.0;
+        if ((e & 3) == 3 && __all(ov || !mine)) break;
+      }
+      if (mine && !ov) ++h;
+    }
+    __builtin_amdgcn_wave_barrier();  // the list is rewritten for the next cell
+  }
+  for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o, 64);
+  if (lane == 0 && h) atomicAdd(hits, h);
+}
+
 // check_overlap(idx, sf, sf, sf) (moves.c:157-212) for each query.
 __global__ void k_overlap_scaled(Grid g, const double *__restrict__ x, const double *__restrict__ y,
                                  const double *__restrict__ z, const int *__restrict__ idx,
@@ -443,7 +545,134 @@ int build_grid(hsmc_gpu_ctx *ctx, int replica, double cut, Grid &g) {
 
 bool bad_replica(const hsmc_gpu_ctx *ctx, int r) { return !ctx || r < 0 || r >= ctx->R; }
 
+// Scale factors of one scan, passed by value (no host-to-device copy).
+struct SfTable {
+  double v[64];
+};
+
+__global__ void k_store_sf(SfTable t, int n, double *__restrict__ dst) {
+  const int k = threadIdx.x;
+  if (k < n) dst[k] = t.v[k];
+}
+
+__global__ void k_zero_words(unsigned int *__restrict__ p, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Workspace layout of the scan: counts, fill counters and starts of the
+// grid's C cells, cell and id of the N spheres, their sorted positions, the
+// scale table and the result mask, each region 256-B aligned.
+struct ScanLayout {
+  size_t cnt, fill, start, cellof, ids, px, py, pz, sf, mask, bytes;
+};
+
+ScanLayout scan_layout(long N, long C) {
+  ScanLayout l{};
+  size_t o = 0;
+  auto take = [&](size_t b) {
+    const size_t at = o;
+    o += align256(b);
+    return at;
+  };
+  l.cnt = take(sizeof(int) * C);
+  l.fill = take(sizeof(int) * C);
+  l.start = take(sizeof(int) * (C + 1));
+  l.cellof = take(sizeof(int) * N);
+  l.ids = take(sizeof(int) * N);
+  l.px = take(sizeof(double) * N);
+  l.py = take(sizeof(double) * N);
+  l.pz = take(sizeof(double) * N);
+  l.sf = take(sizeof(double) * 64);
+  l.mask = take(sizeof(unsigned long long));
+  l.bytes = o;
+  return l;
+}
+
+int scan_cut(const double *sf, int n_sf, double *cut) {
+  double smin = 1.0;
+  for (int k = 0; k < n_sf; ++k) {
+    if (!(sf[k] > 0.0)) return HSMC_ERR_BAD_ARG;
+    smin = std::min(smin, sf[k]);
+  }
+  *cut = (1.0 / smin) * (1.0 + 1e-6);
+  return HSMC_OK;
+}
+
 }  // namespace
+
+namespace hsmc {
+
+size_t overlap_scan_workspace_bytes(const hsmc_gpu_ctx *ctx, int replica, const double *sf, int n_sf) {
+  if (bad_replica(ctx, replica) || n_sf < 1 || n_sf > 64 || !sf) return 0;
+  double cut = 0.0;
+  if (scan_cut(sf, n_sf, &cut)) return 0;
+  const hsmc_gpu_replica &p = ctx->reps[replica].p;
+  const long C = (long)grid_cells_for(p.lx, cut) * grid_cells_for(p.ly, cut) * grid_cells_for(p.lz, cut);
+  return scan_layout(p.n_part, C).bytes;
+}
+
+// The scan of hsmc_gpu_any_overlap_scaled with every buffer carved from the
+// device workspace ws (ws_bytes long): the same kernels, the grid rebuilt from
+// the replica's positions, counters cleared by kernels, the scale factors
+// passed by value.  Nothing is allocated.
+int overlap_scan_in_workspace(hsmc_gpu_ctx *ctx, int replica, const double *sf, int n_sf,
+                              unsigned char *ws, size_t ws_bytes, uint8_t *no_overlap) {
+  if (bad_replica(ctx, replica) || n_sf < 1 || n_sf > 64 || !sf || !no_overlap || !ws)
+    return HSMC_ERR_BAD_ARG;
+  double cut = 0.0;
+  int st = scan_cut(sf, n_sf, &cut);
+  if (st) return st;
+  const RepHost &h = ctx->reps[replica];
+  const int N = h.p.n_part;
+  const int nx = grid_cells_for(h.p.lx, cut), ny = grid_cells_for(h.p.ly, cut),
+            nz = grid_cells_for(h.p.lz, cut);
+  const long C = (long)nx * ny * nz;
+  const ScanLayout l = scan_layout(N, C);
+  if (l.bytes > ws_bytes) {
+    ctx->last_error = "the scan needs " + std::to_string(l.bytes) + " bytes of workspace, " +
+                      std::to_string(ws_bytes) + " available";
+    return HSMC_ERR_NOT_SUPPORTED;
+  }
+  int *cnt = (int *)(ws + l.cnt), *fill = (int *)(ws + l.fill), *start = (int *)(ws + l.start);
+  int *cellof = (int *)(ws + l.cellof), *ids = (int *)(ws + l.ids);
+  double *px = (double *)(ws + l.px), *py = (double *)(ws + l.py), *pz = (double *)(ws + l.pz);
+  double *dsf = (double *)(ws + l.sf);
+  unsigned long long *dmask = (unsigned long long *)(ws + l.mask);
+  Grid g;
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  g.lx = h.p.lx; g.ly = h.p.ly; g.lz = h.p.lz;
+  g.hx = h.p.lx / 2.0; g.hy = h.p.ly / 2.0; g.hz = h.p.lz / 2.0;
+  g.sx = h.p.lx / nx; g.sy = h.p.ly / ny; g.sz = h.p.lz / nz;
+  g.start = start; g.ids = ids; g.px = px; g.py = py; g.pz = pz;
+  // counts and fill counters are adjacent regions: one clearing kernel
+  const long zero_words = (long)((l.start - l.cnt) / sizeof(unsigned int));
+  hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((zero_words + 255) / 256)), dim3(256), 0, ctx->stream,
+                     (unsigned int *)cnt, zero_words);
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, ctx->stream, (unsigned int *)dmask, 2L);
+  SfTable t{};
+  for (int k = 0; k < n_sf; ++k) t.v[k] = sf[k];
+  hipLaunchKernelGGL(k_store_sf, dim3(1), dim3(64), 0, ctx->stream, t, n_sf, dsf);
+  const double *x = ctx->d_x + h.poff, *y = ctx->d_y + h.poff, *z = ctx->d_z + h.poff;
+  const unsigned blocks = (unsigned)((N + 255) / 256);
+  hipLaunchKernelGGL(k_grid_count, dim3(blocks), dim3(256), 0, ctx->stream, x, y, z, N, nx, ny, nz,
+                     g.sx, g.sy, g.sz, cnt, cellof);
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, ctx->stream, cnt, (int)C, start);
+  hipLaunchKernelGGL(k_grid_scatter, dim3(blocks), dim3(256), 0, ctx->stream, x, y, z, N, cellof,
+                     start, fill, ids, px, py, pz);
+  hipLaunchKernelGGL(k_any_overlap_scaled, dim3(blocks), dim3(256), 0, ctx->stream, g, N, dsf, n_sf,
+                     cut * cut, dmask);
+  HSMC_HIP(ctx, hipGetLastError());
+  unsigned long long mask = 0;
+  HSMC_HIP(ctx, hipMemcpyAsync(&mask, dmask, sizeof(mask), hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < n_sf; ++k) no_overlap[k] = ((mask >> k) & 1ull) ? 0 : 1;
+  return HSMC_OK;
+}
+
+}  // namespace hsmc
 
 extern "C" {
 
@@ -714,6 +943,46 @@ int hsmc_gpu_widom_uniform(hsmc_gpu_ctx *ctx, int replica, uint64_t seed, uint64
   HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   *hits = (int64_t)hv;
   return HSMC_OK;
+}
+
+int hsmc_gpu_widom_cells(hsmc_gpu_ctx *ctx, int replica, uint64_t seed, uint64_t sample,
+                         int32_t per_cell, int64_t *hits, int64_t *n_points, int32_t out_grid[3]) {
+  if (bad_replica(ctx, replica) || per_cell < 0 || !hits) return HSMC_ERR_BAD_ARG;
+  *hits = 0;
+  Grid g;
+  int st = build_grid(ctx, replica, 1.0 + 1e-9, g);
+  if (st) return st;
+  const long C = (long)g.nx * g.ny * g.nz;
+  if (n_points) *n_points = (int64_t)per_cell * C;
+  if (out_grid) {
+    out_grid[0] = g.nx;
+    out_grid[1] = g.ny;
+    out_grid[2] = g.nz;
+  }
+  if (per_cell == 0) return HSMC_OK;
+  // spheres of unit diameter centred in a 27-cell stencil fit in a box one
+  // diameter wider: at most the close-packing fraction (0.7405, taken as
+  // 0.78) of its volume over the sphere volume
+  const double span = (std::min(3, g.nx) * g.sx + 1.0) * (std::min(3, g.ny) * g.sy + 1.0) *
+                      (std::min(3, g.nz) * g.sz + 1.0);
+  const long bound = std::min((long)ctx->reps[replica].p.n_part, (long)ceil(0.78 * span / (M_PI / 6.0)));
+  if (bound > kWidomCellsLcap) {
+    ctx->last_error = "observable-grid cells too wide for the stratified insertion kernel";
+    return HSMC_ERR_NOT_SUPPORTED;
+  }
+  unsigned long long *dh = (unsigned long long *)scratch(ctx, S_OUT1, sizeof(unsigned long long));
+  if (!dh) return HSMC_ERR_NO_MEMORY;
+  HSMC_HIP(ctx, hipMemsetAsync(dh, 0, sizeof(unsigned long long), ctx->stream));
+  const long blocks = std::min((C + kWidomCellsWaves - 1) / kWidomCellsWaves, 1L << 20);
+  hipLaunchKernelGGL(k_widom_cells, dim3((unsigned)blocks), dim3(64 * kWidomCellsWaves), 0, ctx->stream, g,
+                     (uint32_t)seed, (uint32_t)(seed >> 32), (unsigned long long)sample, (int)per_cell,
+                     dh, ctx->d_err);
+  HSMC_HIP(ctx, hipGetLastError());
+  unsigned long long hv = 0;
+  HSMC_HIP(ctx, hipMemcpyAsync(&hv, dh, sizeof(hv), hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *hits = (int64_t)hv;
+  return check_device_errors(ctx);
 }
 
 }  // extern "C"

@@ -33,6 +33,7 @@ EXPORTS = [
     "hsmc_gpu_any_overlap_scaled", "hsmc_gpu_pair_dist", "hsmc_gpu_rdf_counts",
     "hsmc_gpu_contact_counts", "hsmc_gpu_widom_points", "hsmc_gpu_widom_uniform",
     "hsmc_gpu_set_particles", "hsmc_gpu_set_dr_max", "hsmc_gpu_occupant_picks",
+    "hsmc_gpu_any_overlap_scaled_noalloc", "hsmc_gpu_widom_cells",
     "hsmc_gpu_mlp_create", "hsmc_gpu_mlp_destroy", "hsmc_gpu_mlp_set", "hsmc_gpu_mlp_get",
     "hsmc_gpu_mlp_forward", "hsmc_gpu_mlp_gradient", "hsmc_gpu_point_descriptors",
     "hsmc_gpu_mlp_train", "hsmc_gpu_voxel_scores", "hsmc_gpu_widom_learned",
@@ -106,11 +107,13 @@ def load() -> C.CDLL:
         "hsmc_gpu_overlap_batch": (i32, [vp, i32, P(i32), P(f64), i32, P(C.c_uint8)]),
         "hsmc_gpu_overlap_scaled": (i32, [vp, i32, P(i32), P(f64), i32, P(C.c_uint8)]),
         "hsmc_gpu_any_overlap_scaled": (i32, [vp, i32, P(f64), i32, P(C.c_uint8)]),
+        "hsmc_gpu_any_overlap_scaled_noalloc": (i32, [vp, i32, P(f64), i32, P(C.c_uint8)]),
         "hsmc_gpu_pair_dist": (i32, [vp, i32, P(i32), P(i32), i32, f64, P(f64)]),
         "hsmc_gpu_rdf_counts": (i32, [vp, i32, f64, i32, f64, P(i64), P(i64)]),
         "hsmc_gpu_contact_counts": (i32, [vp, i32, f64, i32, f64, P(i64), P(i64)]),
         "hsmc_gpu_widom_points": (i32, [vp, i32, P(f64), i32, P(C.c_uint8), P(i64)]),
         "hsmc_gpu_widom_uniform": (i32, [vp, i32, u64, u64, i64, i64, P(i64)]),
+        "hsmc_gpu_widom_cells": (i32, [vp, i32, u64, u64, i32, P(i64), P(i64), P(i32)]),
         "hsmc_gpu_mlp_create": (i32, [vp, P(vp), u64]),
         "hsmc_gpu_mlp_destroy": (i32, [vp]),
         "hsmc_gpu_mlp_set": (i32, [vp, P(C.c_float)]),
@@ -358,6 +361,16 @@ class Context:
         self._ck(self.L.hsmc_gpu_widom_uniform(self.h, r, int(seed), int(sample), int(first),
                                                int(n), C.byref(hits)), "widom_uniform")
         return hits.value
+
+    def widom_cells(self, seed: int, sample: int, per_cell: int, r: int = 0) -> tuple:
+        """Cell-stratified uniform insertion: (hits, points, grid shape)."""
+        hits = C.c_int64(0)
+        npts = C.c_int64(0)
+        grid = np.zeros(3, dtype=np.int32)
+        self._ck(self.L.hsmc_gpu_widom_cells(self.h, r, int(seed), int(sample), int(per_cell),
+                                             C.byref(hits), C.byref(npts), _p(grid, C.c_int32)),
+                 "widom_cells")
+        return hits.value, npts.value, tuple(int(v) for v in grid)
 
 
 class Mlp:

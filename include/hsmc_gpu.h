@@ -179,6 +179,17 @@ int hsmc_gpu_overlap_scaled(hsmc_gpu_ctx *ctx, int replica, const int32_t *idx,
  * (compute_press.c:243-273), as one all-particle reduction. */
 int hsmc_gpu_any_overlap_scaled(hsmc_gpu_ctx *ctx, int replica, const double *sf,
                                 int n_sf, uint8_t *no_overlap);
+/* Allocation-free variant of hsmc_gpu_any_overlap_scaled, with no
+ * counterpart in the reference: for callers that may not allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results; the neighbour grid, scale
+ * table and result mask live in a 16 MiB device array that the library
+ * holds on each GPU until the process ends, so no call allocates.  A replica
+ * whose scan needs more (about 32 B per sphere plus 12 B per grid cell:
+ * N = 256000 needs about 11 MB) gets HSMC_ERR_NOT_SUPPORTED.  Calls on one
+ * GPU share the array: calls that overlap in time on one GPU race. */
+int hsmc_gpu_any_overlap_scaled_noalloc(hsmc_gpu_ctx *ctx, int replica, const double *sf,
+                                        int n_sf, uint8_t *no_overlap);
 /* The same scan for every replica at once: no_overlap[r] = 1 when no pair of
  * replica r overlaps after scaling by sf[r] (replicas with sf[r] <= 0 are
  * not scanned and report 1).  One kernel serves the replicas whose sweep-grid
@@ -210,6 +221,17 @@ int hsmc_gpu_widom_points(hsmc_gpu_ctx *ctx, int replica, const double *xyz, int
  * i = first..first+n-1 of Philox stream (seed, sample); *hits = insertable. */
 int hsmc_gpu_widom_uniform(hsmc_gpu_ctx *ctx, int replica, uint64_t seed,
                            uint64_t sample, int64_t first, int64_t n, int64_t *hits);
+
+/* Cell-stratified uniform insertion: each cell of the replica's insertion
+ * grid (cells of side >= 1, out_grid[3] receives its shape; may be NULL)
+ * gets per_cell points uniform inside it, point j of cell c from Philox
+ * stream (seed, sample) at counter {c, j}.  The cells have equal volume, so
+ * *hits / *n_points estimates the insertion probability without bias, as
+ * widom_insertion's uniform points do (compute_widom_chem_pot.c:44-80); the
+ * overlap test is the same FP64 one.  One wave per cell stages the 27-cell
+ * neighbourhood once, so this is the fast form of uniform Widom. */
+int hsmc_gpu_widom_cells(hsmc_gpu_ctx *ctx, int replica, uint64_t seed, uint64_t sample,
+                         int32_t per_cell, int64_t *hits, int64_t *n_points, int32_t out_grid[3]);
 
 /* ---- learned Widom proposal (no counterpart in the reference, whose
  * insertions are uniform, compute_widom_chem_pot.c:73-80) -----------------

@@ -652,6 +652,35 @@ int64_t orc_widom_philox(const orc_sys *s, uint64_t seed, uint64_t sample,
   return hits;
 }
 
+/* Cell-stratified uniform insertion (no reference counterpart; restates
+ * hsmc_gpu_widom_cells): per_cell points uniform in each cell of an
+ * nx*ny*nz grid of the box, point j of cell c = (ix*ny + iy)*nz + iz from
+ * Philox counter {c, j, sample_lo, 0x5743xxxx}, coordinate
+ * (i + (u + 0.5) 2^-32) * (L/n) per axis; each tested by the reference's
+ * Widom overlap test (compute_widom_chem_pot.c:82-160). */
+int64_t orc_widom_cells(const orc_sys *s, uint64_t seed, uint64_t sample, int nx, int ny,
+                        int nz, int per_cell) {
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  const double sx = s->lx / nx, sy = s->ly / ny, sz = s->lz / nz;
+  int64_t hits = 0;
+  for (int ix = 0; ix < nx; ix++)
+    for (int iy = 0; iy < ny; iy++)
+      for (int iz = 0; iz < nz; iz++) {
+        const uint32_t c = (uint32_t)((ix * ny + iy) * nz + iz);
+        for (int j = 0; j < per_cell; j++) {
+          const uint32_t ctr[4] = {c, (uint32_t)j, (uint32_t)sample,
+                                   0x57430000u | (uint32_t)((sample >> 32) & 0xFFFFu)};
+          uint32_t u[4];
+          orc_philox4x32_10(ctr, key, u);
+          const double x = ((double)ix + ((double)u[0] + 0.5) * 0x1p-32) * sx;
+          const double y = ((double)iy + ((double)u[1] + 0.5) * 0x1p-32) * sy;
+          const double z = ((double)iz + ((double)u[2] + 0.5) * 0x1p-32) * sz;
+          if (!widom_overlap(s, x, y, z)) hits++;
+        }
+      }
+  return hits;
+}
+
 /* ===================================================================== */
 /* Checkerboard sweep: CPU restatement of the HIP kernel's algorithm.     */
 /* (No reference counterpart; the reference sweep is serial, nvt.c:201.)  */

@@ -125,6 +125,9 @@ int64_t orc_widom_insertion(orc_sys *s, int n);                                 
 /* Widom on the Philox stream used by the GPU's uniform insertion kernel */
 int64_t orc_widom_philox(const orc_sys *s, uint64_t seed, uint64_t sample,
                          int64_t first, int64_t n);
+/* cell-stratified uniform insertion (restates hsmc_gpu_widom_cells) */
+int64_t orc_widom_cells(const orc_sys *s, uint64_t seed, uint64_t sample, int nx, int ny,
+                        int nz, int per_cell);
 void orc_widom_philox_point(const orc_sys *s, uint64_t seed, uint64_t sample,
                             int64_t i, double out[3]);
 

@@ -65,6 +65,7 @@ def lib() -> C.CDLL:
             "orc_widom_points": (i64, [vp, P(f64), i32, P(C.c_uint8)]),
             "orc_widom_insertion": (i64, [vp, i32]),
             "orc_widom_philox": (i64, [vp, u64, u64, i64, i64]),
+            "orc_widom_cells": (i64, [vp, u64, u64, i32, i32, i32, i32]),
             "orc_widom_philox_point": (None, [vp, u64, u64, i64, P(f64)]),
             "orc_time_sweeps": (i32, [vp, i32, P(f64), P(f64)]),
             "orc_cb_create": (vp, [i32, P(f64), f64, f64, f64, i32, i32, i32, f64, u64]),
@@ -261,6 +262,11 @@ class System:
 
     def widom_philox(self, seed: int, sample: int, first: int, n: int) -> int:
         return lib().orc_widom_philox(self.h, seed, sample, first, n)
+
+    def widom_cells(self, seed: int, sample: int, grid, per_cell: int) -> int:
+        """Hits of the cell-stratified insertion (restates hsmc_gpu_widom_cells)."""
+        nx, ny, nz = grid
+        return int(lib().orc_widom_cells(self.h, seed, sample, nx, ny, nz, per_cell))
 
     def widom_philox_point(self, seed: int, sample: int, i: int) -> np.ndarray:
         out = np.zeros(3)

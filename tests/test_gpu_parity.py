@@ -393,3 +393,47 @@ def test_occupant_picks_match_oracle_and_the_rule(n):
     e = count / bins
     chi2 = float(np.sum((h - e) ** 2 / e))
     assert chi2 < (bins - 1) + 6 * math.sqrt(2 * max(bins - 1, 1)) + 10, (n, h)
+
+
+@pytest.mark.parametrize("n,rho,per_cell,sweeps", [(5, 0.5, 40, 100), (10, 0.8, 8, 60),
+                                                   (6, 0.93, 16, 30), (30, 0.90, 12, 20)])
+def test_widom_cells_match_oracle(n, rho, per_cell, sweeps):
+    """Cell-stratified uniform insertion: per_cell points in every cell of
+    the insertion grid (side >= 1), hits bit-exact against the oracle's
+    restatement of the same Philox points (each decided by the reference's
+    Widom overlap test), up to N = 108000 at rho = 0.90."""
+    o = O.System(2, n, rho=rho, neigh_dr=1.0, max_part=16, seed=2)
+    conf = o.config()
+    L = o.box()["lx"]
+    ctx = gpu.Context([dict(n=len(conf), box=(L, L, L), dr_max=0.1, seed=2)])
+    ctx.upload(conf)
+    ctx.sweeps(sweeps)
+    conf = ctx.download()
+    o.set_config(conf)
+    hits, npts, grid = ctx.widom_cells(7, 3, per_cell)
+    assert npts == per_cell * grid[0] * grid[1] * grid[2]
+    assert L / grid[0] >= 1.0 and L / (grid[0] + 1) < 1.0 + 1e-9
+    assert hits == o.widom_cells(7, 3, grid, per_cell)
+    if rho <= 0.8:
+        assert hits > 0
+    ctx.close()
+
+
+def test_widom_cells_agrees_with_iid_uniform():
+    """The stratified estimate and plain i.i.d. uniform insertion estimate
+    the same insertion probability (rho = 0.5, N = 4000, 20 samples each)."""
+    o = O.System(2, 10, rho=0.5, neigh_dr=1.0, max_part=16, seed=4)
+    L = o.box()["lx"]
+    ctx = gpu.Context([dict(n=4000, box=(L, L, L), dr_max=0.1, seed=4)])
+    ctx.upload(o.config())
+    ctx.sweeps(200)
+    a, b = [], []
+    for s in range(20):
+        h, n, _ = ctx.widom_cells(11, s, 50)
+        a.append(h / n)
+        b.append(ctx.widom_uniform(13, s, 0, n) / n)
+        ctx.sweeps(20)
+    pa, pb = np.mean(a), np.mean(b)
+    sa, sb = np.std(a, ddof=1) / math.sqrt(20), np.std(b, ddof=1) / math.sqrt(20)
+    assert abs(pa - pb) < 4 * math.hypot(sa, sb), (pa, pb, sa, sb)
+    ctx.close()

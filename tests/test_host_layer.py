@@ -70,3 +70,20 @@ def test_extension_keywords(tmp_path):
     assert gi.gpu_serial == 1 and gi.gpu_device == 0 and gi.seed == 7
     p.write_text("gpu_sweep checkerboard\n")
     assert H.read_input_file(str(p)).gpu_serial == 0
+
+
+CALLER_DIR = os.path.join(ROOT, "tests", "c_caller")
+
+
+def build_caller(out_dir):
+    """Compile the reference-style caller against include/hsmc.h and link it
+    with libhsmc.so (every symbol it names must resolve)."""
+    subprocess.run(["make", "-s", "-C", CALLER_DIR, f"OUT={out_dir}"], check=True, timeout=120)
+    return os.path.join(str(out_dir), "caller")
+
+
+def test_reference_style_caller_links(tmp_path):
+    exe = build_caller(tmp_path)
+    assert os.access(exe, os.X_OK)
+    dyn = subprocess.run(["ldd", exe], capture_output=True, text=True, timeout=60).stdout
+    assert "libhsmc.so" in dyn and "libhsmc_gpu.so" in dyn
