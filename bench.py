@@ -102,13 +102,14 @@ def observables(ctx, no_cpu: bool) -> dict:
 
 
 def traffic_from_profile(tuning: dict):
-    """HBM bytes per sublattice-phase launch at this tuning, from the
-    calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE runs in profiles/ (None when
-    that tuning has not been profiled)."""
+    """HBM bytes per sublattice-phase launch at this launch shape (lanes per
+    cell, cells per workgroup; graph batching does not change a dispatch),
+    from the calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE runs in profiles/
+    (None when that shape has not been profiled)."""
     path = os.path.join(ROOT, "profiles", "sweep_phase_n32000_pmc.json")
     if not os.path.exists(path):
         return None
-    key = f"{tuning['lanes_per_cell']},{tuning['cells_per_block']},{tuning['sweeps_per_graph']}"
+    key = f"{tuning['lanes_per_cell']},{tuning['cells_per_block']}"
     try:
         with open(path) as f:
             entry = json.load(f)["by_tuning"].get(key)
@@ -400,13 +401,22 @@ def _widom_chain(ctx, advance, target, S, gap, n_u, n_l, seed=0, train_points=1_
     ctx.sync()
     loss, t_train = _timed(lambda: net.train(voxel, 777 + seed, 0, train_points, epochs=10,
                                              batch=4096, lr=2e-3, beta=20.0))
+    # uniform insertion in its fast, cell-stratified form: per_cell points in
+    # each cell of the insertion grid, about n_u in all
+    _, _, grid = ctx.widom_cells(0, 0, 0)
+    cells = grid[0] * grid[1] * grid[2]
+    per_cell = max(1, round(n_u / cells))
+    n_u = per_cell * cells
+    # the i.i.d. uniform kernel (widom_insertion's point stream) on the
+    # starting configuration, for its insertion rate
+    _, t_iid = _timed(lambda: ctx.widom_uniform(99, 0, 0, n_u))
     pu, pl, tu, tl = [], [], [], []
     hits_u = hits_l = 0
     t_chain = 0.0
     for k in range(S):
         _, dt = _timed(lambda: (advance(gap), ctx.sync()))
         t_chain += dt
-        h, dt = _timed(lambda: ctx.widom_uniform(1234 + seed, k, 0, n_u))
+        (h, _, _), dt = _timed(lambda: ctx.widom_cells(1234 + seed, k, per_cell))
         pu.append(h / n_u)
         tu.append(dt)
         hits_u += h
@@ -428,6 +438,8 @@ def _widom_chain(ctx, advance, target, S, gap, n_u, n_l, seed=0, train_points=1_
                     ms_per_configuration=1e3 * secs / S, insertions_per_s=n_per * S / secs)
 
     uni = summary(pu, tu, n_u, 0.0, hits_u)
+    uni["form"] = f"cell-stratified, {per_cell} points in each of {cells} cells"
+    uni["iid_kernel_insertions_per_s"] = n_u / t_iid
     lrn = summary(pl, tl, n_l, t_train, hits_l)
     lrn["training_seconds"] = t_train
     lrn["training_loss"] = loss
@@ -482,9 +494,10 @@ def widom_headline(args) -> dict:
     print(f"[bench] widom: learned {res['learned']}", file=sys.stderr, flush=True)
     g = int(box[0] // 1.0)
     res["workload"] = ("N=108000 (fcc 4*30^3) rho=0.90 melted by 10^4 sweeps; NVT chain, one "
-                       "sample every gap sweeps; uniform 1e8 and learned 1e6 insertions per "
-                       "sample; voxel 0.25 sigma (198^3), mix 0.9, network 12-64-64-1 trained on "
-                       "1e6 uniform insertions (training charged to the learned seconds)")
+                       "sample every gap sweeps; uniform (cell-stratified) about 1e8 and learned "
+                       "1e6 insertions per sample; voxel 0.25 sigma (198^3), mix 0.9, network "
+                       "12-64-64-1 trained on 1e6 uniform insertions (training charged to the "
+                       "learned seconds)")
     res["carnahan_starling_beta_mu_ex"] = A.carnahan_starling(0.90)["mu_ex"]
     res["roofline"] = _widom_kernel_roofline(res, 108000, g ** 3)
     return res

@@ -34,6 +34,7 @@ namespace hsmc {
 
 constexpr unsigned kErrOverflow = 1u;  // a cell received more than cap occupants
 constexpr unsigned kErrListFull = 2u;  // an LDS candidate list overflowed
+constexpr unsigned kErrStall = 4u;     // a persistent launch's grid barrier timed out
 
 struct RepDev {
   double lx, ly, lz;     // box
@@ -98,6 +99,9 @@ struct hsmc_gpu_ctx {
   std::vector<unsigned long long> tot_acc, tot_left;
   unsigned long long sweep_at_reset = 0;
   unsigned int *d_err = nullptr;
+  // arrival counter and generation of the persistent sweep's grid barrier
+  unsigned int *d_bar = nullptr;
+  int n_cu = 0;  // compute units of the device (co-resident workgroup bound)
   // pinned host copy of the error bits, refreshed by an asynchronous copy
   // after each batch of sweeps, so later batches are not launched onto a
   // cell list that has already overflowed

@@ -262,6 +262,10 @@ int check_device_errors(hsmc_gpu_ctx *ctx) {
   if (e) {
     unsigned int zero = 0;
     HSMC_HIP(ctx, hipMemcpy(ctx->d_err, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    if (e & kErrStall) {
+      ctx->last_error = "a persistent sweep launch stopped at a grid barrier that did not fill";
+      return HSMC_ERR_HIP;
+    }
     ctx->last_error = (e & kErrOverflow) ? "a cell received more occupants than it holds"
                                          : "an overlap candidate list overflowed";
     return HSMC_ERR_CELL_OVERFLOW;
@@ -370,8 +374,11 @@ int hsmc_gpu_create(hsmc_gpu_ctx **out, int device, int n_replicas,
       hipMalloc(&ctx->d_y, sizeof(double) * ptot) != hipSuccess ||
       hipMalloc(&ctx->d_z, sizeof(double) * ptot) != hipSuccess ||
       hipMalloc(&ctx->d_sweep, sizeof(unsigned long long)) != hipSuccess ||
-      hipMalloc(&ctx->d_err, sizeof(unsigned int)) != hipSuccess)
+      hipMalloc(&ctx->d_err, sizeof(unsigned int)) != hipSuccess ||
+      hipMalloc(&ctx->d_bar, 2 * sizeof(unsigned int)) != hipSuccess)
     return fail(HSMC_ERR_NO_MEMORY);
+  if (hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+    return fail(HSMC_ERR_HIP);
   {
     void *pinned = nullptr;
     if (hipHostMalloc(&pinned, sizeof(unsigned int), hipHostMallocDefault) != hipSuccess)
@@ -383,6 +390,7 @@ int hsmc_gpu_create(hsmc_gpu_ctx **out, int device, int n_replicas,
       hipMemset(ctx->d_sweep, 0, sizeof(unsigned long long)) != hipSuccess ||
       hipMemset(ctx->d_swp, 0, sizeof(SweepDev) * n_replicas) != hipSuccess ||
       hipMemset(ctx->d_err, 0, sizeof(unsigned int)) != hipSuccess ||
+      hipMemset(ctx->d_bar, 0, 2 * sizeof(unsigned int)) != hipSuccess ||
       hipMemset(ctx->d_x, 0, sizeof(double) * ptot) != hipSuccess ||
       hipMemset(ctx->d_y, 0, sizeof(double) * ptot) != hipSuccess ||
       hipMemset(ctx->d_z, 0, sizeof(double) * ptot) != hipSuccess)
@@ -399,7 +407,7 @@ int hsmc_gpu_destroy(hsmc_gpu_ctx *ctx) {
   drop_graph(ctx);
   void *bufs[] = {ctx->d_reps, ctx->d_swp, ctx->d_aoff, ctx->d_arep, ctx->d_prep, ctx->d_x, ctx->d_y, ctx->d_z,
                   ctx->d_cnt, ctx->d_slot, ctx->d_sweep,
-                  ctx->d_cacc, ctx->d_cleft, ctx->d_err};
+                  ctx->d_cacc, ctx->d_cleft, ctx->d_err, ctx->d_bar};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (int i = 0; i < hsmc_gpu_ctx::kScratch; ++i)

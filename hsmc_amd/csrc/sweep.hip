@@ -70,14 +70,16 @@ __device__ __forceinline__ uint32_t sweep_order(const RepDev &rp, unsigned long 
   return pk;
 }
 
-__global__ void __launch_bounds__(256)
-k_rebin(const RepDev *__restrict__ reps, const RepDev rp0, SweepDev *__restrict__ swp, int R,
-        const int *__restrict__ prep, int P, int C,
-        const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
-        int *__restrict__ cnt, SlotRec *__restrict__ slot, int cap,
-        const unsigned long long *__restrict__ sweep_base, int k, unsigned *__restrict__ err) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const unsigned long long sweep = *sweep_base + (unsigned long long)(long long)k;
+// Work item i of a sweep's rebin: clear cell i of the next sweep's count
+// buffer, draw replica i's shift and sublattice order, and file particle i
+// into its cell of the shifted grid.
+__device__ __forceinline__ void rebin_one(int i, const RepDev *__restrict__ reps, const RepDev &rp0,
+                                          SweepDev *__restrict__ swp, int R,
+                                          const int *__restrict__ prep, int P, int C,
+                                          const double *__restrict__ x, const double *__restrict__ y,
+                                          const double *__restrict__ z, int *__restrict__ cnt,
+                                          SlotRec *__restrict__ slot, int cap,
+                                          unsigned long long sweep, unsigned *__restrict__ err) {
   const int buf = (int)(sweep & 1ull);
   if (i < C) cnt[(size_t)(buf ^ 1) * C + i] = 0;  // next sweep's buffer
   if (i < R) {  // replica i's shift and sublattice order, read by this sweep's phases
@@ -111,6 +113,17 @@ k_rebin(const RepDev *__restrict__ reps, const RepDev rp0, SweepDev *__restrict_
   }
 }
 
+__global__ void __launch_bounds__(256)
+k_rebin(const RepDev *__restrict__ reps, const RepDev rp0, SweepDev *__restrict__ swp, int R,
+        const int *__restrict__ prep, int P, int C,
+        const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+        int *__restrict__ cnt, SlotRec *__restrict__ slot, int cap,
+        const unsigned long long *__restrict__ sweep_base, int k, unsigned *__restrict__ err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned long long sweep = *sweep_base + (unsigned long long)(long long)k;
+  rebin_one(i, reps, rp0, swp, R, prep, P, C, x, y, z, cnt, slot, cap, sweep, err);
+}
+
 // Value of v in lane t of the wave, for a t that is the same in every lane:
 // v_readlane, an SGPR read with no LDS round trip.  Used when a lane group is
 // the whole wave; with two 32-lane groups per wave the halves need different
@@ -130,40 +143,37 @@ using dvec = double __attribute__((ext_vector_type(M)));
 // Occupancy: at least 6 waves per SIMD (at most 80 VGPRs) for stencil lists
 // of up to 192 entries, 4 for the longest lists and for one replica (whose
 // few active cells never fill the device).
+// One active cell's attempts in one sublattice phase: the work of one group
+// of LPC lanes.  a is the active-cell index, a0 the first active cell of the
+// group's workgroup (of cells_per_block consecutive cells), Lm the group's
+// LDS region.
 template <int CAP, int LPC, int LCAP, bool ONE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!ONE && LCAP <= 192 ? 6 : 4)))
-k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__restrict__ swp,
-        const int *__restrict__ arep, int R, int A, int phase,
-        int k, const unsigned long long *__restrict__ sweep_base, double *__restrict__ x,
-        double *__restrict__ y, double *__restrict__ z, const int *__restrict__ cnt, int C,
-        SlotRec *__restrict__ slot, unsigned int *__restrict__ cacc,
-        unsigned int *__restrict__ cleft, unsigned int *__restrict__ err) {
+__device__ __forceinline__ void phase_cell(
+    int a, int a0, int lane, int *Lm, const RepDev *__restrict__ reps, const RepDev &rp0,
+    const SweepDev *__restrict__ swp, const int *__restrict__ arep, int A, int phase,
+    unsigned long long sweep, double *__restrict__ x, double *__restrict__ y,
+    double *__restrict__ z, const int *__restrict__ cnt, int C, SlotRec *__restrict__ slot,
+    unsigned int *__restrict__ cacc, unsigned int *__restrict__ cleft,
+    unsigned int *__restrict__ err) {
   // The stencil list (the spheres of the 27 neighbour cells, at most LCAP, a
   // bound the host derives from the cell side) lives in registers: entry e
   // is held by lane e % LPC as its copy e / LPC.  LDS holds the slot of each
   // entry, the particle ids, id order and positions of the active cell's
   // occupants, and (32-lane groups) each attempt's displacement.
   constexpr int M = (LCAP + LPC - 1) / LPC;
-  extern __shared__ __attribute__((aligned(16))) int smem_i[];
-  const int cpb = blockDim.x / LPC;
-  const int lane = threadIdx.x & (LPC - 1);
-  const int sub = threadIdx.x / LPC;
-  int *Lm = smem_i + (size_t)sub * (LCAP + 14 * CAP);  // slot of each list entry
   int *Sp = Lm + LCAP;                                 // ids of the active cell's occupants
   int *Lo = Sp + CAP;                                  // occupants in ascending id order
   double *Op = reinterpret_cast<double *>(Lo + CAP);   // occupants' positions (x, y, z)
   double *Od = Op + 3 * CAP;                           // each attempt's displacement
 
-  const int a = blockIdx.x * cpb + sub;
   if (a >= A) return;  // the whole lane group leaves together
-  const unsigned long long sweep = *sweep_base + (unsigned long long)k;
   const int *cntb = cnt + (size_t)(sweep & 1ull) * C;
   // the replica's record (a kernel argument when there is one replica) and
   // the sweep's shift and sublattice order, which the rebin drew.  Each
   // replica's active cells are padded to a multiple of 8 and cells_per_block
   // divides 8, so a workgroup lies in one replica: r is uniform and the
   // record and table are scalar loads.
-  const int r = ONE ? 0 : arep[blockIdx.x * cpb];
+  const int r = ONE ? 0 : arep[a0];
   const RepDev rp = ONE ? rp0 : reps[r];
   const SweepDev sw = swp[r];
   const int colour = (int)((sw.order >> (4 * phase)) & 0xFu);
@@ -406,6 +416,106 @@ k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__res
   }
 }
 
+// Occupancy: at least 6 waves per SIMD (at most 80 VGPRs) for stencil lists
+// of up to 192 entries, 4 for the longest lists and for one replica (whose
+// few active cells never fill the device).
+template <int CAP, int LPC, int LCAP, bool ONE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!ONE && LCAP <= 192 ? 6 : 4)))
+k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__restrict__ swp,
+        const int *__restrict__ arep, int R, int A, int phase,
+        int k, const unsigned long long *__restrict__ sweep_base, double *__restrict__ x,
+        double *__restrict__ y, double *__restrict__ z, const int *__restrict__ cnt, int C,
+        SlotRec *__restrict__ slot, unsigned int *__restrict__ cacc,
+        unsigned int *__restrict__ cleft, unsigned int *__restrict__ err) {
+  extern __shared__ __attribute__((aligned(16))) int smem_i[];
+  const int cpb = blockDim.x / LPC;
+  const int lane = threadIdx.x & (LPC - 1);
+  const int sub = threadIdx.x / LPC;
+  phase_cell<CAP, LPC, LCAP, ONE>(blockIdx.x * cpb + sub, blockIdx.x * cpb, lane,
+                                  smem_i + (size_t)sub * (LCAP + 14 * CAP), reps, rp0, swp, arep, A,
+                                  phase, *sweep_base + (unsigned long long)k, x, y, z, cnt, C, slot,
+                                  cacc, cleft, err);
+}
+
+// Barrier of a cooperative launch (every workgroup resident): an arrival
+// counter and a generation word.  The wait is bounded: a workgroup that
+// waits too long, or sees another's stall, sets kErrStall and the barrier
+// returns false, after which every workgroup leaves the kernel, so a launch
+// always drains.
+struct GridBarrier {
+  unsigned count, gen;
+};
+
+__device__ bool grid_barrier(GridBarrier *__restrict__ b, unsigned *__restrict__ err) {
+  __shared__ int ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ok = 1;
+    __threadfence();  // this workgroup's stores before its arrival
+    const unsigned g = __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned arrived =
+        __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (arrived == gridDim.x - 1) {
+      __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&b->gen, g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      unsigned spins = 0;
+      while (__hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        if (++spins > (1u << 22) ||
+            (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kErrStall)) {
+          atomicOr(err, kErrStall);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __threadfence();  // the other workgroups' stores before this one goes on
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+// Persistent sweeps: one cooperative launch runs n_sweeps whole sweeps, each
+// the rebin's work (every thread of the grid strides over particles and
+// cells) and the 8 sublattice phases (each workgroup strides over its share
+// of the active cells), with a grid barrier after each of the 9 steps in
+// place of a kernel boundary.  The result is the same as the launch-per-step
+// sequence: the same work items, the same order of steps, the same Philox
+// streams.
+template <int CAP, int LPC, int LCAP, bool ONE>
+__global__ void __launch_bounds__(256)
+k_sweeps_persistent(const RepDev *__restrict__ reps, const RepDev rp0, SweepDev *__restrict__ swp,
+                    const int *__restrict__ arep, const int *__restrict__ prep, int R, int P, int C,
+                    int A, double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
+                    int *__restrict__ cnt, SlotRec *__restrict__ slot, int cap,
+                    const unsigned long long *__restrict__ sweep_base, int n_sweeps,
+                    unsigned int *__restrict__ cacc, unsigned int *__restrict__ cleft,
+                    unsigned int *__restrict__ err, GridBarrier *__restrict__ bar) {
+  extern __shared__ __attribute__((aligned(16))) int smem_i[];
+  const int cpb = blockDim.x / LPC;
+  const int lane = threadIdx.x & (LPC - 1);
+  const int sub = threadIdx.x / LPC;
+  int *Lm = smem_i + (size_t)sub * (LCAP + 14 * CAP);
+  const int work = P > C ? P : C;
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gstride = gridDim.x * blockDim.x;
+  const unsigned long long base = *sweep_base;
+  for (int s = 0; s < n_sweeps; ++s) {
+    const unsigned long long sweep = base + (unsigned long long)s;
+    for (int i = gtid; i < work; i += gstride)
+      rebin_one(i, reps, rp0, swp, R, prep, P, C, x, y, z, cnt, slot, cap, sweep, err);
+    if (!grid_barrier(bar, err)) return;
+    for (int p = 0; p < 8; ++p) {
+      for (int a0 = blockIdx.x * cpb; a0 < A; a0 += gridDim.x * cpb) {
+        wave_lds_sync();  // the group's LDS region is reused cell after cell
+        phase_cell<CAP, LPC, LCAP, ONE>(a0 + sub, a0, lane, Lm, reps, rp0, swp, arep, A, p, sweep, x,
+                                        y, z, cnt, C, slot, cacc, cleft, err);
+      }
+      if (!grid_barrier(bar, err)) return;
+    }
+  }
+}
+
 __global__ void k_advance(unsigned long long *sweep_base, int n) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *sweep_base += (unsigned long long)n;
 }
@@ -445,6 +555,77 @@ PhaseFn phase_fn(int cap, int lpc, int lcap, bool one) {
   return one ? phase_fn_o<true>(cap, lpc, lcap) : phase_fn_o<false>(cap, lpc, lcap);
 }
 
+using PersistentFn = void (*)(const RepDev *, const RepDev, SweepDev *, const int *, const int *,
+                              int, int, int, int, double *, double *, double *, int *, SlotRec *,
+                              int, const unsigned long long *, int, unsigned int *,
+                              unsigned int *, unsigned int *, GridBarrier *);
+
+template <int CAP, int LCAP, bool ONE>
+PersistentFn persistent_fn_l(int lpc) {
+  return lpc == 64 ? k_sweeps_persistent<CAP, 64, LCAP, ONE> : k_sweeps_persistent<CAP, 32, LCAP, ONE>;
+}
+
+template <bool ONE>
+PersistentFn persistent_fn_o(int cap, int lpc, int lcap) {
+  if (cap == 8) {
+    if (lcap <= 128) return persistent_fn_l<8, 128, ONE>(lpc);
+    return lcap <= 192 ? persistent_fn_l<8, 192, ONE>(lpc) : persistent_fn_l<8, 256, ONE>(lpc);
+  }
+  if (lcap <= 128) return persistent_fn_l<16, 128, ONE>(lpc);
+  if (lcap <= 192) return persistent_fn_l<16, 192, ONE>(lpc);
+  return persistent_fn_l<16, 256, ONE>(lpc);
+}
+
+PersistentFn persistent_fn(int cap, int lpc, int lcap, bool one) {
+  return one ? persistent_fn_o<true>(cap, lpc, lcap) : persistent_fn_o<false>(cap, lpc, lcap);
+}
+
+__global__ void k_zero_barrier(GridBarrier *b) {
+  if (threadIdx.x == 0) {
+    b->count = 0u;
+    b->gen = 0u;
+  }
+}
+
+// One cooperative launch of n sweeps: as many workgroups as the device
+// holds at once (the launch fails rather than oversubscribe), each striding
+// over its share of particles and active cells; then the counter advance.
+int launch_persistent(hsmc_gpu_ctx *ctx, int n) {
+  const int lpc = ctx->tune.lanes_per_cell, cpb = ctx->tune.cells_per_block;
+  const size_t lds = phase_lds_bytes(ctx->cap, ctx->lcap, cpb);
+  PersistentFn fn = persistent_fn(ctx->cap, lpc, ctx->lcap, ctx->R == 1);
+  int per_cu = 0;
+  HSMC_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, lpc * cpb, lds));
+  const long need = (ctx->Atot + cpb - 1) / cpb;
+  const long grid = std::min(need, (long)per_cu * ctx->n_cu);
+  if (grid < 1) {
+    ctx->last_error = "the persistent sweep kernel does not fit on a compute unit";
+    return HSMC_ERR_NOT_SUPPORTED;
+  }
+  // the barrier starts empty (a launch that stopped at a stalled barrier
+  // leaves it partly filled); cleared by a kernel, not a memset
+  hipLaunchKernelGGL(k_zero_barrier, dim3(1), dim3(64), 0, ctx->stream,
+                     reinterpret_cast<GridBarrier *>(ctx->d_bar));
+  const RepDev *reps = ctx->d_reps;
+  RepDev rp0 = ctx->rep0;
+  SweepDev *swp = ctx->d_swp;
+  const int *arep = ctx->d_arep, *prep = ctx->d_prep;
+  int R = ctx->R, P = (int)ctx->Ptot, C = (int)ctx->Ctot, A = (int)ctx->Atot, cap = ctx->cap;
+  double *x = ctx->d_x, *y = ctx->d_y, *z = ctx->d_z;
+  int *cnt = ctx->d_cnt;
+  SlotRec *slot = ctx->d_slot;
+  const unsigned long long *sweep = ctx->d_sweep;
+  int ns = n;
+  unsigned int *cacc = ctx->d_cacc, *cleft = ctx->d_cleft, *err = ctx->d_err;
+  GridBarrier *bar = reinterpret_cast<GridBarrier *>(ctx->d_bar);
+  void *args[] = {&reps, &rp0, &swp, &arep, &prep, &R, &P, &C, &A, &x, &y, &z, &cnt, &slot,
+                  &cap, &sweep, &ns, &cacc, &cleft, &err, &bar};
+  HSMC_HIP(ctx, hipLaunchCooperativeKernel((const void *)fn, dim3((unsigned)grid), dim3(lpc * cpb),
+                                           args, (unsigned)lds, ctx->stream));
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, ctx->stream, ctx->d_sweep, n);
+  HSMC_HIP(ctx, hipGetLastError());
+  return HSMC_OK;
+}
 
 bool tuning_ok(const hsmc_gpu_ctx *ctx, const hsmc_gpu_tuning &t) {
   if (t.lanes_per_cell != 32 && t.lanes_per_cell != 64) return false;
@@ -455,6 +636,7 @@ bool tuning_ok(const hsmc_gpu_ctx *ctx, const hsmc_gpu_tuning &t) {
     return false;
   if (phase_lds_bytes(ctx->cap, ctx->lcap, t.cells_per_block) > 64 * 1024) return false;
   if (t.sweeps_per_graph < 0 || t.sweeps_per_graph > 512) return false;
+  if (t.persistent != 0 && t.persistent != 1) return false;
   return true;
 }
 
@@ -530,6 +712,18 @@ int run_sweeps(hsmc_gpu_ctx *ctx, int n) {
   if (n <= 0) return HSMC_OK;
   if (*ctx->h_err) return check_device_errors(ctx);
   ctx->version++;
+  if (ctx->tune.persistent) {  // cooperative launches of whole sweeps
+    const int per = ctx->tune.sweeps_per_graph > 0 ? ctx->tune.sweeps_per_graph : 64;
+    for (int done = 0; done < n; done += per) {
+      if (*ctx->h_err) return check_device_errors(ctx);
+      int st = launch_persistent(ctx, std::min(per, n - done));
+      if (st) return st;
+      HSMC_HIP(ctx, hipMemcpyAsync((void *)ctx->h_err, ctx->d_err, sizeof(unsigned int),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    }
+    ctx->cells_version = ctx->version;
+    return HSMC_OK;
+  }
   const int K = ctx->tune.sweeps_per_graph;
   int done = 0;
   if (K > 0 && n >= K) {
@@ -602,7 +796,8 @@ int hsmc_gpu_time_sweeps(hsmc_gpu_ctx *ctx, int n_sweeps, float *ms) {
   // capture outside the timed region: the batch graph, and a graph of
   // exactly the remaining n mod sweeps_per_graph sweeps (the whole call when
   // it is shorter than a batch), so every timed sweep is a graph replay
-  const int K = ctx->tune.sweeps_per_graph;
+  // (persistent launches capture nothing)
+  const int K = ctx->tune.persistent ? 0 : ctx->tune.sweeps_per_graph;
   if (K > 0 && n_sweeps >= K &&
       (!ctx->gexec || ctx->graph_sweeps != K || !same_launch_shape(ctx->graph_tune, ctx->tune))) {
     int st = capture_graph(ctx, K);

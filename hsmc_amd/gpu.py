@@ -51,11 +51,11 @@ class Replica(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("lanes_per_cell", C.c_int32), ("cells_per_block", C.c_int32),
-                ("sweeps_per_graph", C.c_int32), ("reserved", C.c_int32)]
+                ("sweeps_per_graph", C.c_int32), ("persistent", C.c_int32)]
 
     def as_dict(self) -> dict:
         return dict(lanes_per_cell=self.lanes_per_cell, cells_per_block=self.cells_per_block,
-                    sweeps_per_graph=self.sweeps_per_graph)
+                    sweeps_per_graph=self.sweeps_per_graph, persistent=self.persistent)
 
 
 class HsmcError(RuntimeError):
@@ -269,8 +269,9 @@ class Context:
     def set_sweep_counter(self, sweep: int) -> None:
         self._ck(self.L.hsmc_gpu_set_sweep_counter(self.h, sweep), "set_sweep_counter")
 
-    def set_tuning(self, lanes_per_cell: int, cells_per_block: int, sweeps_per_graph: int) -> None:
-        t = Tuning(lanes_per_cell, cells_per_block, sweeps_per_graph, 0)
+    def set_tuning(self, lanes_per_cell: int, cells_per_block: int, sweeps_per_graph: int,
+                   persistent: int = 0) -> None:
+        t = Tuning(lanes_per_cell, cells_per_block, sweeps_per_graph, persistent)
         self._ck(self.L.hsmc_gpu_set_tuning(self.h, C.byref(t)), "set_tuning")
 
     def get_tuning(self) -> dict:

@@ -59,12 +59,16 @@ typedef struct hsmc_gpu_replica {
  * hand).  lanes_per_cell: 32 or 64 lanes test one active cell;
  * cells_per_block: active cells per workgroup, 1, 2, 4 or 8 (at most 256
  * lanes); sweeps_per_graph: sweeps captured in one hipGraph launch (0 =
- * direct launches, at most 512).  Other values give HSMC_ERR_BAD_ARG. */
+ * direct launches, at most 512).  persistent = 1: instead, one cooperative
+ * launch of resident workgroups runs sweeps_per_graph whole sweeps (64 when
+ * 0), with a grid barrier between the rebin and each sublattice phase in
+ * place of a kernel boundary; the trajectory is the same.  Other values
+ * give HSMC_ERR_BAD_ARG. */
 typedef struct hsmc_gpu_tuning {
   int32_t lanes_per_cell;
   int32_t cells_per_block;
   int32_t sweeps_per_graph;
-  int32_t reserved;
+  int32_t persistent;
 } hsmc_gpu_tuning;
 
 /* ---- life cycle -------------------------------------------------------- */

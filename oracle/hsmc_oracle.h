@@ -12,18 +12,30 @@
  * evaluation order, cell-list layout and RNG semantics are kept.
  *
  * Pinning status (how each group is checked, see tests/test_oracle_*.py):
- *   - MT19937 core: C++11 std::mt19937 known-answer (10000th output of seed
- *     5489 is 4123659995).  GSL's seed-0 rule and uniform_int rejection rule
- *     follow GSL's published documentation; GSL is absent here, so the exact
- *     GSL stream is "parity unpinned".
+ *   - MT19937 core and seeding: C++11 std::mt19937 known-answer (10000th
+ *     output of seed 5489 is 4123659995), and the raw stream of numpy's own
+ *     MT19937 (init_genrand seeding) for several seeds.
+ *   - rng_get_double: draw / (2^32 - 1) against that independent stream.
+ *   - rng_get_int (GSL uniform_int): against that stream with n chosen so a
+ *     quarter to a third of all draws fall in the discarded tail (outputs are
+ *     exactly the in-range scaled draws, one draw consumed per discard), and
+ *     by monotone, equal-width intervals and chi-square uniformity for small
+ *     n.  GSL's seed-0 rule (seed 0 -> 4357) stays "parity unpinned": GSL is
+ *     absent and no check here exercises seed 0.
+ *   - vol_move's acceptance (moves.c:117-121): the ideal-gas closed form
+ *     <V> = (N+1)/beta P of a chain in which no pair can overlap, which an
+ *     N or N+2 exponent misses by about 13 standard errors.
  *   - Philox4x32-10: Random123 known-answer vectors.
  *   - box / lattice / cell list / compute_dist / check_overlap / part_move
  *     building blocks / rdf / contact / presst / widom: golden vectors written
  *     by the reference's own translation units compiled from their sources
  *     (oracle/ref/), plus brute-force and closed-form checks.
- *   - checkerboard sweep (orc_cb_*): restates the HIP sweep kernel's own
- *     algorithm (it has no reference counterpart); pinned by invariants
- *     (no overlaps, particle conservation, detailed-balance statistics).
+ *   - checkerboard sweep (orc_cb_*) and cell-stratified insertion
+ *     (orc_widom_cells): restate the HIP kernels' own algorithms (no reference
+ *     counterpart); pinned by invariants (no overlaps, particle conservation,
+ *     detailed-balance statistics), by agreement with the reference-pinned
+ *     i.i.d. Widom estimate, and the occupant pick (orc_cb_pick) by range and
+ *     uniformity at n where half the draws are redrawn.
  */
 #ifndef HSMC_ORACLE_H
 #define HSMC_ORACLE_H

@@ -221,3 +221,21 @@ def test_full_c3_batch_properties_and_oracle_replicas():
         c = ctx.counters(r)
         assert (c["attempts"], c["accepted"], c["left_cell"]) == \
             tuple(int(v) for v in cb.counters()[:3]), r
+
+
+def test_persistent_batched_sweeps_equal_launched_sweeps():
+    """The persistent launch over a batch of uneven replicas equals the
+    graph-launched sweeps bit for bit, replica by replica."""
+    reps = _replicas(12, 31, 3, 7)
+    outs = []
+    for persistent in (0, 1):
+        ctx = gpu.Context(reps)
+        for r, rep in enumerate(reps):
+            ctx.upload(rep["conf"], r)
+        ctx.set_tuning(32, 4, 6 if persistent else 3, persistent)
+        ctx.sweeps(15)
+        outs.append([(ctx.download(r), ctx.counters(r)) for r in range(len(reps))])
+        ctx.close()
+    for r in range(len(reps)):
+        assert np.array_equal(outs[0][r][0], outs[1][r][0]), r
+        assert outs[0][r][1] == outs[1][r][1], r

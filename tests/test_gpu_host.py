@@ -401,22 +401,22 @@ def test_reference_style_caller_runs(tmp_path):
     from tests.test_host_layer import build_caller
     exe = build_caller(tmp_path)
     (tmp_path / "in.dat").write_text(
-        "rho 0.5\ncells_x 3\ncells_y 3\ncells_z 3\ntype 2\nneigh_list 1.05 10\ndr_max 0.1\n"
+        "rho 0.5\ncells_x 4\ncells_y 4\ncells_z 4\ntype 2\nneigh_list 1.05 10\ndr_max 0.1\n"
         "opt 0 1000 10 0.5 0.5\npress_virial 0.01 10\nwidom 1000 10\nconfig_write 100 1\n"
         "seed 3\nsweep_eq 0\nsweep_stat 300\nout 100\n")
     r = subprocess.run([exe, "in.dat"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     line = r.stdout.strip().splitlines()[-1].split()
     vals = dict(zip(line[0::2], (int(v) for v in line[1::2])))
-    assert vals["N"] == 108 and vals["attempts"] == 108 * 300
+    assert vals["N"] == 256 and vals["attempts"] == 256 * 300
     assert vals["overlaps"] == 0 and vals["outside"] == 0
     assert 0.3 < vals["accepted"] / vals["attempts"] < 1.0
     pv = (tmp_path / "press_virial.dat").read_text().splitlines()
-    assert pv[0].startswith("####") and pv[3].split()[0] == "5" and pv[3].split()[2] == "108"
+    assert pv[0].startswith("####") and pv[3].split()[0] == "5" and pv[3].split()[2] == "256"
     mu = [l for l in (tmp_path / "chem_pot.dat").read_text().splitlines() if not l.startswith("#")]
     assert len(mu) == 30
     frac = np.array([float(l.split()[1]) for l in mu])
     assert 0.0 < frac.mean() < 1.0
     with gzip.open(tmp_path / "config_000000.dat.gz", "rt") as f:
         text = f.read().splitlines()
-    assert text[0] == "# Sweep number" and len(text) > 108
+    assert text[0] == "# Sweep number" and len(text) > 256
