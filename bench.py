@@ -168,7 +168,7 @@ def main() -> None:
                     help="leave the Widom comparison out of the headline line")
     ap.add_argument("--widom-configs", type=int, default=128,
                     help="configurations sampled along the Widom chain")
-    ap.add_argument("--widom-gap", type=int, default=50, help="sweeps between Widom samples")
+    ap.add_argument("--widom-gap", type=int, default=100, help="sweeps between Widom samples")
     ap.add_argument("--profile-widom", action="store_true",
                     help="only run the Widom kernels on the rho = 0.90 system (for rocprofv3)")
     args = ap.parse_args()
@@ -424,6 +424,10 @@ def _widom_chain(ctx, advance, target, S, gap, n_u, n_l, seed=0, train_points=1_
         pl.append(r["estimate"])
         tl.append(dt)
         hits_l += r["hits"]
+    # the variance per insertion on one fixed configuration (the last): 8
+    # repeated estimates per method, so configurational variance is absent
+    rep_u = [ctx.widom_cells(777 + k, 0, per_cell)[0] / n_u for k in range(8)]
+    rep_l = [net.widom(voxel, mix, 888 + k, 0, n_l)["estimate"] for k in range(8)]
     net.close()
 
     def summary(p, t, n_per, fixed, hits):
@@ -431,10 +435,13 @@ def _widom_chain(ctx, advance, target, S, gap, n_u, n_l, seed=0, train_points=1_
         mean = float(p.mean())
         sig = A.blocking_error(p) / mean if mean > 0 else float("inf")
         secs = float(sum(t))
+        factor = (sig / target) ** 2
         return dict(beta_mu=-math.log(mean) if mean > 0 else float("inf"), sigma=sig,
                     p_insert=mean, configurations=S, insertions=n_per * S, hits=int(hits),
                     seconds_measured=fixed + secs,
-                    seconds_to_target=fixed + secs * (sig / target) ** 2,
+                    seconds_to_target=fixed + secs * factor,
+                    seconds_to_target_with_chain=fixed + (secs + t_chain) * factor,
+                    blocking_relative=(A.blocking_std(p)[:, 0] / mean).tolist() if mean > 0 else [],
                     ms_per_configuration=1e3 * secs / S, insertions_per_s=n_per * S / secs)
 
     uni = summary(pu, tu, n_u, 0.0, hits_u)
@@ -443,10 +450,14 @@ def _widom_chain(ctx, advance, target, S, gap, n_u, n_l, seed=0, train_points=1_
     lrn = summary(pl, tl, n_l, t_train, hits_l)
     lrn["training_seconds"] = t_train
     lrn["training_loss"] = loss
+    var_u = float(np.var(rep_u, ddof=1)) * n_u
+    var_l = float(np.var(rep_l, ddof=1)) * n_l
     return {
         "ratio": uni["seconds_to_target"] / lrn["seconds_to_target"],
+        "ratio_with_chain": uni["seconds_to_target_with_chain"] / lrn["seconds_to_target_with_chain"],
         "agreement_sigmas": abs(uni["beta_mu"] - lrn["beta_mu"]) / math.hypot(uni["sigma"], lrn["sigma"]),
         "target_sigma": target,
+        "insertion_variance_gain_fixed_configuration": var_u / var_l if var_l > 0 else None,
         "uniform": uni, "learned": lrn,
         "chain": {"gap": gap, "steps": S * gap, "seconds": t_chain},
     }
@@ -482,20 +493,22 @@ def _widom_kernel_roofline(res, n_part, grid_cells):
 def widom_headline(args) -> dict:
     """The Widom half of the headline metric: GPU seconds to sigma(beta
     mu_ex) = 0.01 at rho = 0.90, N = 108000, learned against uniform, along
-    one running NVT chain (10^4 melt sweeps, then --widom-configs samples
+    one running NVT chain (4*10^4 melt sweeps, then --widom-configs samples
     --widom-gap sweeps apart)."""
     from hsmc_amd import analysis as A
-    ctx, box = _widom_system(0.90, 10000, args.tuning)
+    ctx, box = _widom_system(0.90, 40000, args.tuning)
     print(f"[bench] widom: N=108000 rho=0.90 melted, box {box[0]:.4f}", file=sys.stderr, flush=True)
     res = _widom_chain(ctx, ctx.sweeps, args.sigma, args.widom_configs, args.widom_gap,
-                       100_000_000, 1_000_000)
+                       10_000_000, 250_000)
     ctx.close()
     print(f"[bench] widom: uniform {res['uniform']}", file=sys.stderr, flush=True)
     print(f"[bench] widom: learned {res['learned']}", file=sys.stderr, flush=True)
     g = int(box[0] // 1.0)
-    res["workload"] = ("N=108000 (fcc 4*30^3) rho=0.90 melted by 10^4 sweeps; NVT chain, one "
-                       "sample every gap sweeps; uniform (cell-stratified) about 1e8 and learned "
-                       "1e6 insertions per sample; voxel 0.25 sigma (198^3), mix 0.9, network "
+    res["workload"] = ("N=108000 (fcc 4*30^3) rho=0.90 melted by 4*10^4 sweeps; NVT chain, one "
+                       "sample every gap sweeps; uniform (cell-stratified) about 1e7 and learned "
+                       "2.5e5 insertions per sample (the insertion probability of a configuration "
+                       "varies by about 30% from one sample to the next, so larger per-sample "
+                       "budgets buy little); voxel 0.25 sigma (198^3), mix 0.9, network "
                        "12-64-64-1 trained on 1e6 uniform insertions (training charged to the "
                        "learned seconds)")
     res["carnahan_starling_beta_mu_ex"] = A.carnahan_starling(0.90)["mu_ex"]
@@ -510,6 +523,7 @@ def profile_widom(args) -> None:
     ctx, _ = _widom_system(0.90, 2000, args.tuning)
     for k in range(3):
         ctx.widom_uniform(5, k, 0, 100_000_000)
+        ctx.widom_cells(6, k, 850)   # about 1e8 points, cell-stratified
     net = gpu.Mlp(ctx, seed=5)
     net.train(0.25, 7, 0, 100_000, epochs=1, batch=4096, lr=2e-3, beta=20.0)
     for k in range(3):

@@ -6,6 +6,7 @@ loading fails loudly when the library has not been built.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import weakref
@@ -65,6 +66,18 @@ class HsmcError(RuntimeError):
 
 
 _lib = None
+# contexts not yet closed: released, networks first, by an exit hook that runs
+# before the interpreter tears down module state, so no device object is
+# destroyed during interpreter finalization
+_live = weakref.WeakSet()
+
+
+def _close_live() -> None:
+    for ctx in list(_live):
+        ctx.close()
+
+
+atexit.register(_close_live)
 
 
 def load() -> C.CDLL:
@@ -162,6 +175,7 @@ class Context:
         self.replicas = [dict(r) for r in replicas]
         self.R = len(replicas)
         self._nets = weakref.WeakSet()   # networks to destroy before the context
+        _live.add(self)
 
     def close(self) -> None:
         if getattr(self, "h", None):
