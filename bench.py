@@ -171,6 +171,14 @@ def main() -> None:
     ap.add_argument("--widom-gap", type=int, default=100, help="sweeps between Widom samples")
     ap.add_argument("--profile-widom", action="store_true",
                     help="only run the Widom kernels on the rho = 0.90 system (for rocprofv3)")
+    ap.add_argument("--state", type=str, default="",
+                    help="melted configuration (.npy): loaded in place of the melt when the file "
+                         "exists, else written after the melt (profiled runs load one, so the "
+                         "profiler sees only the timed sweeps)")
+    ap.add_argument("--save-only", action="store_true",
+                    help="with --state: melt, write the file and exit")
+    ap.add_argument("--no-observables", action="store_true",
+                    help="leave the per-sample observable timings out")
     args = ap.parse_args()
     defaults = {"n32000": (20000, 1000), "batched": (2000, 200), "widom": (0, 0), "npt": (20, 5),
                 "widom_npt": (2000, 20000)}
@@ -201,8 +209,11 @@ def main() -> None:
     ctx = gpu.Context([dict(n=len(conf), box=box, dr_max=0.1, seed=1)])
     ctx.upload(conf)
     print(f"[bench] grid {ctx.grid()}  N={len(conf)}", file=sys.stderr, flush=True)
-    ctx.sweeps(SETUP_SWEEPS)   # melt the lattice; the autotuner restores this state
-    ctx.sync()
+    if not _load_state(ctx, args.state):
+        ctx.sweeps(SETUP_SWEEPS)   # melt the lattice; the autotuner restores this state
+        ctx.sync()
+        if _save_state(ctx, args):
+            return
     if args.tuning:
         ctx.set_tuning(*(int(v) for v in args.tuning.split(",")))
         tuning = ctx.get_tuning()
@@ -257,7 +268,8 @@ def main() -> None:
                      "launches_timed": launches,
                      "algorithmic_bytes_per_launch": per_launch_bytes},
     }
-    line["observables"] = observables(ctx, args.no_cpu)
+    if not args.no_observables:
+        line["observables"] = observables(ctx, args.no_cpu)
     ctx.close()
     if not args.no_widom:
         line["widom"] = widom_headline(args)
@@ -516,11 +528,38 @@ def widom_headline(args) -> dict:
     return res
 
 
+def _load_state(ctx, path) -> bool:
+    """Upload a configuration written by _save_state, if the file exists."""
+    import numpy as np
+    if not path or not os.path.exists(path):
+        return False
+    ctx.upload(np.load(path))
+    print(f"[bench] configuration loaded from {path}", file=sys.stderr, flush=True)
+    return True
+
+
+def _save_state(ctx, args) -> bool:
+    """Write the melted configuration to --state; True when the run should
+    stop there (--save-only)."""
+    import numpy as np
+    if args.state:
+        np.save(args.state, ctx.download())
+        print(f"[bench] configuration written to {args.state}", file=sys.stderr, flush=True)
+    return bool(args.state and args.save_only)
+
+
 def profile_widom(args) -> None:
     """Only the Widom kernels on the rho = 0.90, N = 108000 system (a few
     calls each), for rocprofv3 counter runs."""
     from hsmc_amd import gpu
-    ctx, _ = _widom_system(0.90, 2000, args.tuning)
+    if args.state and os.path.exists(args.state):
+        ctx, _ = _widom_system(0.90, 0, args.tuning)
+        _load_state(ctx, args.state)
+    else:
+        ctx, _ = _widom_system(0.90, 2000, args.tuning)
+        if _save_state(ctx, args):
+            ctx.close()
+            return
     for k in range(3):
         ctx.widom_uniform(5, k, 0, 100_000_000)
         ctx.widom_cells(6, k, 850)   # about 1e8 points, cell-stratified
