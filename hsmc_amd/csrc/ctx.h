@@ -174,6 +174,11 @@ int overlap_scan_in_workspace(hsmc_gpu_ctx *ctx, int replica, const double *sf, 
                               unsigned char *ws, size_t ws_bytes, uint8_t *no_overlap);
 // Look up the allocation-free scratch array's address on `device` (current).
 int noalloc_scratch_init(int device);
+// Exclusive prefix sum of in[0..n) into out[0..n] (out[n] = total), three
+// launches on stream s; part is a device workspace of scan_workspace_ints(n)
+// ints.
+long scan_workspace_ints(long n);
+void exclusive_scan(hipStream_t s, const int *in, int n, int *out, int *part);
 
 }  // namespace hsmc
 

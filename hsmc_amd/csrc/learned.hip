@@ -723,29 +723,6 @@ __global__ void __launch_bounds__(256) k_sum_fixed(const double *__restrict__ pa
   if (threadIdx.x == 0) *out = s[0];
 }
 
-// single-block exclusive scan of up to a few thousand entries (+ total)
-__global__ void __launch_bounds__(1024) k_scan_small(const int *__restrict__ in, int n,
-                                                     int *__restrict__ out) {
-  __shared__ int part[1024];
-  const int t = threadIdx.x, per = (n + 1023) / 1024, b = t * per, e = min(n, b + per);
-  int s = 0;
-  for (int i = b; i < e; ++i) s += in[i];
-  part[t] = s;
-  __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
-    const int v = t >= d ? part[t - d] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  int run = part[t] - s;
-  for (int i = b; i < e; ++i) {
-    out[i] = run;
-    run += in[i];
-  }
-  if (t == 1023) out[n] = part[1023];
-}
-
 // Build a CSR grid of one replica (side >= cut) into the given scratch slots.
 int build_ngrid(hsmc_gpu_ctx *ctx, int replica, double cut, int slot0, NGrid &g);
 
@@ -806,12 +783,13 @@ int grid_cells_for2(double L, double cut) {
   return n;
 }
 
-// scratch slots 24..39 belong to this file (observables.hip uses 0..12,
+// scratch slots 24..39 belong to this file (observables.hip uses 0..13,
 // ctx.hip 21..22, sweep.hip 23): descriptor grid 24..26, overlap grid 27..29,
-// rows / scores 30, targets / block partials 31, counts 32, offsets 33,
-// sums 34, per-block insertion weights 35, scan partials 36..37.
+// rows / scores 30, targets / block partials 31, cumulative mass 32,
+// sums 34, per-block insertion weights 35, cumulative-mass block sums 36,
+// grid-scan partials 37, voxel scores 38, score block reductions 39.
 enum { L_GRID_A = 24, L_GRID_B = 27, L_X = 30, L_T = 31, L_CDF = 32, L_ACC = 34, L_PART = 35,
-       L_BSUM = 36, L_F = 38, L_FRED = 39 };
+       L_BSUM = 36, L_SCANP = 37, L_F = 38, L_FRED = 39 };
 
 int build_ngrid(hsmc_gpu_ctx *ctx, int replica, double cut, int slot0, NGrid &g) {
   const RepHost &h = ctx->reps[replica];
@@ -825,7 +803,8 @@ int build_ngrid(hsmc_gpu_ctx *ctx, int replica, double cut, int slot0, NGrid &g)
   int *cnt = (int *)scratch(ctx, slot0 + 0, sizeof(int) * 2 * (C + 1) + sizeof(int) * N);
   double *pos = (double *)scratch(ctx, slot0 + 1, sizeof(double) * 3 * N);
   int *start = (int *)scratch(ctx, slot0 + 2, sizeof(int) * (C + 1));
-  if (!cnt || !pos || !start) return HSMC_ERR_NO_MEMORY;
+  int *part = (int *)scratch(ctx, L_SCANP, sizeof(int) * scan_workspace_ints(C));
+  if (!cnt || !pos || !start || !part) return HSMC_ERR_NO_MEMORY;
   int *fill = cnt + (C + 1);
   int *cellof = fill + (C + 1);
   const double *x = ctx->d_x + h.poff, *y = ctx->d_y + h.poff, *z = ctx->d_z + h.poff;
@@ -840,7 +819,7 @@ int build_ngrid(hsmc_gpu_ctx *ctx, int replica, double cut, int slot0, NGrid &g)
   HSMC_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(int) * 2 * (C + 1), ctx->stream));
   hipLaunchKernelGGL(k_grid_count2, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, x, y, z, N,
                      nx, ny, nz, sx, sy, sz, cnt, cellof);
-  hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, ctx->stream, cnt, (int)C, start);
+  exclusive_scan(ctx->stream, cnt, (int)C, start, part);
   hipLaunchKernelGGL(k_grid_scatter2, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, x, y, z,
                      N, cellof, start, fill, pos, pos + N, pos + 2 * (size_t)N);
   HSMC_HIP(ctx, hipGetLastError());

@@ -21,7 +21,8 @@ using namespace hsmc;
 
 namespace {
 
-enum Slot { S_CNT = 0, S_START, S_CELLOF, S_IDS, S_PX, S_PY, S_PZ, S_FILL, S_Q0, S_Q1, S_Q2, S_OUT0, S_OUT1 };
+enum Slot { S_CNT = 0, S_START, S_CELLOF, S_IDS, S_PX, S_PY, S_PZ, S_FILL, S_Q0, S_Q1, S_Q2, S_OUT0, S_OUT1,
+            S_PART };
 
 struct Grid {
   int nx, ny, nz;
@@ -78,6 +79,55 @@ __global__ void __launch_bounds__(1024) k_scan1(const int *__restrict__ cnt, int
     carry += total;
   }
   if (t == 0) start[n] = carry;
+}
+
+// Exclusive prefix sum over many workgroups: each 256-thread workgroup scans
+// a tile of 4096 entries (16 consecutive per thread) and writes the tile
+// total; the totals are scanned by k_scan1; k_scan_add adds each tile's
+// offset.  Three short launches in place of one workgroup walking the whole
+// array.
+constexpr int kScanTile = 4096;
+
+__global__ void __launch_bounds__(256) k_scan_tile(const int *__restrict__ in, int n,
+                                                   int *__restrict__ out, int *__restrict__ part) {
+  __shared__ int wsum[4];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const long base = (long)blockIdx.x * kScanTile + (long)t * 16;
+  int v[16];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0;
+    s += v[k];
+  }
+  int incl = s;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += u;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  int woff = 0;
+  for (int i = 0; i < w; ++i) woff += wsum[i];
+  int run = woff + incl - s;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (t == 255) part[blockIdx.x] = woff + incl;
+}
+
+__global__ void k_scan_empty(int *__restrict__ out) {
+  if (threadIdx.x == 0) out[0] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_scan_add(int *__restrict__ out, int n,
+                                                  const int *__restrict__ offs, int tiles) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += offs[i / kScanTile];
+  if (i == 0) out[n] = offs[tiles];
 }
 
 __global__ void k_grid_scatter(const double *__restrict__ x, const double *__restrict__ y,
@@ -494,6 +544,31 @@ k_pair_hist_tiled(Grid g, int nbins, double dr_bin, double rmax_eff, double cut2
   if (lane == 0 && bad) atomicAdd(oor, bad);
 }
 
+}  // namespace
+
+namespace hsmc {
+
+long scan_workspace_ints(long n) {
+  const long tiles = (n + kScanTile - 1) / kScanTile;
+  return 2 * tiles + 1;
+}
+
+void exclusive_scan(hipStream_t s, const int *in, int n, int *out, int *part) {
+  const int tiles = (int)((n + kScanTile - 1) / kScanTile);
+  if (tiles == 0) {
+    hipLaunchKernelGGL(k_scan_empty, dim3(1), dim3(64), 0, s, out);
+    return;
+  }
+  int *offs = part + tiles;  // tiles + 1 entries: exclusive offsets and the total
+  hipLaunchKernelGGL(k_scan_tile, dim3((unsigned)tiles), dim3(256), 0, s, in, n, out, part);
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, part, tiles, offs);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out, n, offs, tiles);
+}
+
+}  // namespace hsmc
+
+namespace {
+
 int grid_cells_for(double L, double cut) {
   int n = (int)floor(L / cut);
   if (n < 1) n = 1;
@@ -517,7 +592,9 @@ int build_grid(hsmc_gpu_ctx *ctx, int replica, double cut, Grid &g) {
   double *px = (double *)scratch(ctx, S_PX, sizeof(double) * N);
   double *py = (double *)scratch(ctx, S_PY, sizeof(double) * N);
   double *pz = (double *)scratch(ctx, S_PZ, sizeof(double) * N);
-  if (!cnt || !fill || !start || !cellof || !ids || !px || !py || !pz) return HSMC_ERR_NO_MEMORY;
+  int *part = (int *)scratch(ctx, S_PART, sizeof(int) * scan_workspace_ints(C));
+  if (!cnt || !fill || !start || !cellof || !ids || !px || !py || !pz || !part)
+    return HSMC_ERR_NO_MEMORY;
   g.nx = nx; g.ny = ny; g.nz = nz;
   g.lx = h.p.lx; g.ly = h.p.ly; g.lz = h.p.lz;
   g.hx = h.p.lx / 2.0; g.hy = h.p.ly / 2.0; g.hz = h.p.lz / 2.0;
@@ -532,7 +609,7 @@ int build_grid(hsmc_gpu_ctx *ctx, int replica, double cut, Grid &g) {
   const unsigned blocks = (unsigned)((N + 255) / 256);
   hipLaunchKernelGGL(k_grid_count, dim3(blocks), dim3(256), 0, ctx->stream, x, y, z, N, nx, ny, nz,
                      g.sx, g.sy, g.sz, cnt, cellof);
-  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, ctx->stream, cnt, (int)C, start);
+  exclusive_scan(ctx->stream, cnt, (int)C, start, part);
   hipLaunchKernelGGL(k_grid_scatter, dim3(blocks), dim3(256), 0, ctx->stream, x, y, z, N, cellof,
                      start, fill, ids, px, py, pz);
   HSMC_HIP(ctx, hipGetLastError());
@@ -566,7 +643,7 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 // grid's C cells, cell and id of the N spheres, their sorted positions, the
 // scale table and the result mask, each region 256-B aligned.
 struct ScanLayout {
-  size_t cnt, fill, start, cellof, ids, px, py, pz, sf, mask, bytes;
+  size_t cnt, fill, start, cellof, ids, px, py, pz, sf, mask, part, bytes;
 };
 
 ScanLayout scan_layout(long N, long C) {
@@ -587,6 +664,7 @@ ScanLayout scan_layout(long N, long C) {
   l.pz = take(sizeof(double) * N);
   l.sf = take(sizeof(double) * 64);
   l.mask = take(sizeof(unsigned long long));
+  l.part = take(sizeof(int) * scan_workspace_ints(C));
   l.bytes = o;
   return l;
 }
@@ -659,7 +737,7 @@ int overlap_scan_in_workspace(hsmc_gpu_ctx *ctx, int replica, const double *sf, 
   const unsigned blocks = (unsigned)((N + 255) / 256);
   hipLaunchKernelGGL(k_grid_count, dim3(blocks), dim3(256), 0, ctx->stream, x, y, z, N, nx, ny, nz,
                      g.sx, g.sy, g.sz, cnt, cellof);
-  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, ctx->stream, cnt, (int)C, start);
+  exclusive_scan(ctx->stream, cnt, (int)C, start, (int *)(ws + l.part));
   hipLaunchKernelGGL(k_grid_scatter, dim3(blocks), dim3(256), 0, ctx->stream, x, y, z, N, cellof,
                      start, fill, ids, px, py, pz);
   hipLaunchKernelGGL(k_any_overlap_scaled, dim3(blocks), dim3(256), 0, ctx->stream, g, N, dsf, n_sf,
@@ -695,7 +773,8 @@ int hsmc_gpu_cell_list(hsmc_gpu_ctx *ctx, int replica, int nx, int ny, int nz, i
     double *py = (double *)scratch(ctx, S_PY, sizeof(double) * N);
     double *pz = (double *)scratch(ctx, S_PZ, sizeof(double) * N);
     int *rows = (int *)scratch(ctx, S_OUT0, sizeof(int) * C * max_part);
-    if (!cnt || !fill || !start || !cellof || !ids || !px || !py || !pz || !rows)
+    int *part = (int *)scratch(ctx, S_PART, sizeof(int) * scan_workspace_ints(C));
+    if (!cnt || !fill || !start || !cellof || !ids || !px || !py || !pz || !rows || !part)
       return HSMC_ERR_NO_MEMORY;
     ctx->aux_replica = -1;  // the CSR scratch now holds this grid
     const double *x = ctx->d_x + h.poff, *y = ctx->d_y + h.poff, *z = ctx->d_z + h.poff;
@@ -704,7 +783,7 @@ int hsmc_gpu_cell_list(hsmc_gpu_ctx *ctx, int replica, int nx, int ny, int nz, i
     const unsigned blocks = (unsigned)((N + 255) / 256);
     hipLaunchKernelGGL(k_grid_count, dim3(blocks), dim3(256), 0, ctx->stream, x, y, z, N, nx, ny,
                        nz, sx, sy, sz, cnt, cellof);
-    hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, ctx->stream, cnt, (int)C, start);
+    exclusive_scan(ctx->stream, cnt, (int)C, start, part);
     hipLaunchKernelGGL(k_grid_scatter, dim3(blocks), dim3(256), 0, ctx->stream, x, y, z, N, cellof,
                        start, fill, ids, px, py, pz);
     hipLaunchKernelGGL(k_part_cell, dim3((unsigned)((C + 255) / 256)), dim3(256), 0, ctx->stream,
