@@ -333,6 +333,26 @@ def test_anisotropic_box_sweeps_and_observables():
     assert np.array_equal(hits.astype(bool), ref)
     assert ref.sum() > 20
 
+    # cell-stratified insertion on the anisotropic insertion grid, each
+    # point decided again by brute force over all spheres
+    per_cell = 12
+    h, n, grid = ctx.widom_cells(3, 1, per_cell)
+    assert len(set(grid)) == 3 and n == per_cell * grid[0] * grid[1] * grid[2]
+    side = np.array(box) / np.array(grid)
+    pts = []
+    for ix in range(grid[0]):
+        for iy in range(grid[1]):
+            for iz in range(grid[2]):
+                c = (ix * grid[1] + iy) * grid[2] + iz
+                for j in range(per_cell):
+                    u = O.philox([c, j, 1, 0x57430000], [3, 0])
+                    pts.append([(i + (u[k] + 0.5) * 2.0 ** -32) * side[k]
+                                for k, i in enumerate((ix, iy, iz))])
+    pts = np.array(pts)
+    free = np.concatenate([~(_min_image_s2(pts[i:i + 10000], p, box) < 1.0).any(axis=1)
+                           for i in range(0, len(pts), 10000)])
+    assert h == int(free.sum()) and h > 0
+
 
 @pytest.mark.parametrize("tuning", [(32, 2, 0), (64, 2, 0), (32, 1, 4)])
 def test_wide_cell_single_replica_matches_oracle(tuning):
