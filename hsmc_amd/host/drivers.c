@@ -275,9 +275,9 @@ void hs_npt(void) {   /* npt.c:25-104 */
 
 /* ---- optimizer (optimizer.c:23-148) ---- */
 
-void get_sample_nvt(double *dr, double *acc_ratio, int sample_iter) {
+void get_sample_nvt(double *dr, double *acc_ratio, int sweeps) {
   reset_moves_counters();
-  run_sweeps(sample_iter, 0);
+  run_sweeps(sweeps, 0);
   uint64_t c[6];
   get_moves_counters64(c);
   *dr = G_IN.dr_max;
@@ -302,40 +302,10 @@ static double secant_step(double x2, double x1, double a2, double a1, double tar
   return isfinite(x) ? x : x2;
 }
 
-void opt_nvt(void) {
-  int sample_iter = G_IN.opt_sweeps / G_IN.opt_samples;
-  double acc_ratio_1, acc_ratio_2, dr_1, dr_2;
-  printf("---------------------------------------------------\n");
-  printf("Maximum displacement optimization started ...\n");
-  printf("Sweeps for optimization: %d\n", G_IN.opt_sweeps);
-  printf("Number of samples: %d\n", G_IN.opt_samples);
-  get_sample_nvt(&dr_1, &acc_ratio_1, sample_iter);
-  if (acc_ratio_1 > G_IN.opt_part_target) G_IN.dr_max *= 2;
-  else G_IN.dr_max /= 2;
-  set_gpu_dr_max();
-  get_sample_nvt(&dr_2, &acc_ratio_2, sample_iter);
-  for (int ii = 0; ii < G_IN.opt_samples; ii++) {
-    G_IN.dr_max = secant_step(dr_2, dr_1, acc_ratio_2, acc_ratio_1, G_IN.opt_part_target);
-    if (G_IN.dr_max > 1.0) {
-      G_IN.dr_max = 1.0;
-    } else if (G_IN.dr_max <= 0.0) {
-      G_IN.dr_max = -G_IN.dr_max;
-      if (G_IN.dr_max > 1.0) G_IN.dr_max = dr_2 / 2;
-    }
-    dr_1 = dr_2;
-    acc_ratio_1 = acc_ratio_2;
-    set_gpu_dr_max();
-    get_sample_nvt(&dr_2, &acc_ratio_2, sample_iter);
-  }
-  printf("Optimal maximum displacement: %.8f\n", G_IN.dr_max);
-  printf("Acceptance ratio: %.8f\n", acc_ratio_2);
-  printf("Maximum displacement optimization completed\n");
-}
-
 void get_sample_npt(double *dr, double *acc_ratio_part, double *dv, double *acc_ratio_vol,
-                    int sample_iter) {
+                    int sweeps) {
   reset_moves_counters();
-  run_sweeps(sample_iter, 1);
+  run_sweeps(sweeps, 1);
   uint64_t c[6];
   get_moves_counters64(c);
   *dr = G_IN.dr_max;
@@ -344,42 +314,82 @@ void get_sample_npt(double *dr, double *acc_ratio_part, double *dv, double *acc_
   *acc_ratio_vol = (double)c[4] / ((double)c[3]);
 }
 
-void opt_npt(void) {
-  int sample_iter = G_IN.opt_sweeps / G_IN.opt_samples;
-  double ap1, ap2, dr_1, dr_2, av1, av2, dv_1, dv_2;
+/* A step width the optimizer tunes toward a target acceptance: the width
+ * itself (G_IN.dr_max or G_IN.dv_max), the last two (width, acceptance)
+ * samples, an upper bound (0: none), and the largest width a reflected
+ * negative step may keep before it falls back to half the last width. */
+typedef struct {
+  double *width;
+  double target;
+  double upper;
+  double reflect_max;
+  double w_prev, acc_prev, w_last, acc_last;
+} tuned_width;
+
+/* Next width from the last two samples (optimizer.c:47-56, :117-130). */
+static void advance_width(tuned_width *t) {
+  double w = secant_step(t->w_last, t->w_prev, t->acc_last, t->acc_prev, t->target);
+  if (t->upper > 0.0 && w > t->upper) {
+    w = t->upper;
+  } else if (w <= 0.0) {
+    w = -w;
+    if (w > t->reflect_max) w = t->w_last / 2;
+  }
+  *t->width = w;
+}
+
+/* Run one sample of sweeps at the current widths and record, for each tuned
+ * width, the width used and the acceptance it gave.  One width tunes the
+ * particle displacement alone (NVT); two add the volume step (NpT). */
+static void sample_widths(tuned_width *t, int n, int sweeps) {
+  double acc[2];
+  if (n == 1)
+    get_sample_nvt(&t[0].w_last, &acc[0], sweeps);
+  else
+    get_sample_npt(&t[0].w_last, &acc[0], &t[1].w_last, &acc[1], sweeps);
+  for (int k = 0; k < n; k++) t[k].acc_last = acc[k];
+}
+
+/* The optimizer's schedule (optimizer.c:23-148): one sample at the input
+ * widths, each width doubled if its acceptance is above target and halved
+ * otherwise, a second sample, then opt_samples secant updates, each followed
+ * by a sample.  opt_sweeps is split evenly over the samples. */
+static void tune_widths(tuned_width *t, int n) {
+  const int sweeps = G_IN.opt_sweeps / G_IN.opt_samples;
   printf("---------------------------------------------------\n");
   printf("Maximum displacement optimization started ...\n");
   printf("Sweeps for optimization: %d\n", G_IN.opt_sweeps);
   printf("Number of samples: %d\n", G_IN.opt_samples);
-  get_sample_npt(&dr_1, &ap1, &dv_1, &av1, sample_iter);
-  if (ap1 > G_IN.opt_part_target) G_IN.dr_max *= 2;
-  else G_IN.dr_max /= 2;
-  if (av1 > G_IN.opt_vol_target) G_IN.dv_max *= 2;
-  else G_IN.dv_max /= 2;
-  set_gpu_dr_max();
-  get_sample_npt(&dr_2, &ap2, &dv_2, &av2, sample_iter);
-  for (int ii = 0; ii < G_IN.opt_samples; ii++) {
-    G_IN.dr_max = secant_step(dr_2, dr_1, ap2, ap1, G_IN.opt_part_target);
-    G_IN.dv_max = secant_step(dv_2, dv_1, av2, av1, G_IN.opt_vol_target);
-    if (G_IN.dr_max > 1.0) G_IN.dr_max = 1.0;
-    if (G_IN.dr_max <= 0.0) {
-      G_IN.dr_max = -G_IN.dr_max;
-      if (G_IN.dr_max > 1.0) G_IN.dr_max = dr_2 / 2;
+  sample_widths(t, n, sweeps);
+  for (int k = 0; k < n; k++)
+    *t[k].width = t[k].acc_last > t[k].target ? *t[k].width * 2 : *t[k].width / 2;
+  for (int round = 0; round <= G_IN.opt_samples; round++) {
+    if (round > 0)
+      for (int k = 0; k < n; k++) advance_width(&t[k]);
+    for (int k = 0; k < n; k++) {
+      t[k].w_prev = t[k].w_last;
+      t[k].acc_prev = t[k].acc_last;
     }
-    if (G_IN.dv_max <= 0.0) {
-      G_IN.dv_max = -G_IN.dv_max;
-      if (G_IN.dv_max > 0.1) G_IN.dv_max = dv_2 / 2;
-    }
-    dr_1 = dr_2;
-    dv_1 = dv_2;
-    ap1 = ap2;
-    av1 = av2;
     set_gpu_dr_max();
-    get_sample_npt(&dr_2, &ap2, &dv_2, &av2, sample_iter);
+    sample_widths(t, n, sweeps);
   }
+}
+
+void opt_nvt(void) {
+  tuned_width t[1] = {{&G_IN.dr_max, G_IN.opt_part_target, 1.0, 1.0, 0, 0, 0, 0}};
+  tune_widths(t, 1);
+  printf("Optimal maximum displacement: %.8f\n", G_IN.dr_max);
+  printf("Acceptance ratio: %.8f\n", t[0].acc_last);
+  printf("Maximum displacement optimization completed\n");
+}
+
+void opt_npt(void) {
+  tuned_width t[2] = {{&G_IN.dr_max, G_IN.opt_part_target, 1.0, 1.0, 0, 0, 0, 0},
+                      {&G_IN.dv_max, G_IN.opt_vol_target, 0.0, 0.1, 0, 0, 0, 0}};
+  tune_widths(t, 2);
   printf("Optimal maximum particle displacement: %.8f\n", G_IN.dr_max);
-  printf("Acceptance ratio: %.8f \n", ap2);
+  printf("Acceptance ratio: %.8f \n", t[0].acc_last);
   printf("Optimal maximum volume deformation: %.8f\n", G_IN.dv_max);
-  printf("Acceptance ratio: %.8f \n", av2);
+  printf("Acceptance ratio: %.8f \n", t[1].acc_last);
   printf("Maximum displacement optimization completed\n");
 }

@@ -309,6 +309,128 @@ out 100
         assert (tmp_path / name).exists(), name
 
 
+def _secant(x2, x1, a2, a1, target):
+    """The optimizer's secant step (optimizer.c:47, :117-118); when the two
+    acceptances are equal the step keeps its last value, as the host layer
+    does in place of the reference's division by zero."""
+    if a2 == a1:
+        return x2
+    x = x2 - (a2 - target) * (x2 - x1) / (a2 - a1)
+    return x if math.isfinite(x) else x2
+
+
+def test_serial_nvt_optimizer_follows_reference_schedule(tmp_path):
+    """Serial NVT with the step optimizer on: the printed optimal dr_max and
+    acceptance, and the production counters after the lattice is rebuilt,
+    equal the oracle's serial chain driven by the schedule of optimizer.c:
+    23-64 (a sample at the input width, double or halve, a second sample,
+    then opt_samples secant steps with dr_max > 1 cut to 1 and a negative
+    step reflected, or halved from the last width when the reflection
+    exceeds 1) and nvt.c:58-68 (lattice and cell list rebuilt after the
+    optimizer, counters reset, RNG stream continued)."""
+    out = run_cli(tmp_path, """rho 0.5
+cells_x 5
+cells_y 5
+cells_z 5
+type 2
+neigh_list 1.05 10
+dr_max 0.1
+opt 1 400 8 0.45 0.5
+seed 7
+sweep_eq 30
+sweep_stat 30
+out 1000
+gpu_sweep serial
+""")
+    o = O.System(2, 5, rho=0.5, neigh_dr=1.05, max_part=10, dr_max=0.1, seed=7)
+    lattice = o.config().copy()
+    per, target = 400 // 8, 0.45
+
+    def sample(dr):
+        o.reset_counters()
+        o.set_params(dr)
+        o.sweep_nvt(per)
+        c = o.counters()
+        return dr, float(c[1]) / float(c[0])
+
+    dr1, a1 = sample(0.1)
+    dr2, a2 = sample(0.1 * 2 if a1 > target else 0.1 / 2)
+    for _ in range(8):
+        dr = _secant(dr2, dr1, a2, a1, target)
+        if dr > 1.0:
+            dr = 1.0
+        elif dr <= 0.0:
+            dr = -dr
+            if dr > 1.0:
+                dr = dr2 / 2
+        dr1, a1 = dr2, a2
+        dr2, a2 = sample(dr)
+    assert "Optimal maximum displacement: %.8f\n" % dr2 in out, out[-1500:]
+    assert "Acceptance ratio: %.8f\n" % a2 in out
+    o.set_config(lattice)
+    o.reset_counters()
+    o.sweep_nvt(60)
+    c = o.counters()
+    assert "-- Particle moves: %.8e" % float(c[0]) in out
+    assert "   Acceptance percentage: %f" % (float(c[1]) / float(c[0])) in out
+
+
+def test_serial_npt_optimizer_follows_reference_schedule(tmp_path):
+    """Serial NpT with the step optimizer on: both tuned widths and their
+    acceptances equal the oracle's serial chain driven by the schedule of
+    optimizer.c:83-148 (dr_max as in NVT; dv_max doubled or halved, then
+    secant steps with a negative step reflected, or halved from the last
+    width when the reflection exceeds 0.1).  About 20 volume moves per
+    sample make equal volume acceptances likely along the way."""
+    out = run_cli(tmp_path, """rho 0.5
+cells_x 5
+cells_y 5
+cells_z 5
+type 2
+neigh_list 1.2 10
+dr_max 0.15
+npt 3.0 0.01
+opt 1 200 10 0.5 0.4
+seed 3
+sweep_eq 10
+sweep_stat 10
+out 1000
+gpu_sweep serial
+""")
+    o = O.System(2, 5, rho=0.5, neigh_dr=1.2, max_part=10, dr_max=0.15, seed=3)
+    per, tp, tv = 200 // 10, 0.5, 0.4
+
+    def sample(dr, dv):
+        o.reset_counters()
+        o.set_params(dr, dv, 3.0)
+        o.sweep_npt(per)
+        c = [float(v) for v in o.counters()]
+        return dr, c[1] / c[0], dv, c[4] / c[3]
+
+    dr1, a1, dv1, v1 = sample(0.15, 0.01)
+    dr2, a2, dv2, v2 = sample(0.15 * 2 if a1 > tp else 0.15 / 2,
+                              0.01 * 2 if v1 > tv else 0.01 / 2)
+    for _ in range(10):
+        dr = _secant(dr2, dr1, a2, a1, tp)
+        dv = _secant(dv2, dv1, v2, v1, tv)
+        if dr > 1.0:
+            dr = 1.0
+        if dr <= 0.0:
+            dr = -dr
+            if dr > 1.0:
+                dr = dr2 / 2
+        if dv <= 0.0:
+            dv = -dv
+            if dv > 0.1:
+                dv = dv2 / 2
+        dr1, a1, dv1, v1 = dr2, a2, dv2, v2
+        dr2, a2, dv2, v2 = sample(dr, dv)
+    assert "Optimal maximum particle displacement: %.8f\n" % dr2 in out, out[-1500:]
+    assert "Acceptance ratio: %.8f \n" % a2 in out
+    assert "Optimal maximum volume deformation: %.8f\n" % dv2 in out
+    assert "Acceptance ratio: %.8f \n" % v2 in out
+
+
 def test_checkerboard_npt_matches_serial_density(tmp_path):
     """NpT at beta P = 3 (N = 256): mean density of the checkerboard sweep
     followed by Binomial(N, 1/(N+1)) volume moves against the reference's
