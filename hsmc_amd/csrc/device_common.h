@@ -135,6 +135,26 @@ __host__ __device__ __forceinline__ bool stencil_axis_ok(int d, int g) {
   return d == 0;
 }
 
+// Offset j in [0, 27) of the 27-cell stencil in order of distance class
+// |dx| + |dy| + |dz| (own cell, faces, edges, corners; lexicographic within
+// a class), for scans that can stop at the first sphere found near a point.
+__host__ __device__ __forceinline__ int stencil_by_distance(int j, int &dx, int &dy, int &dz) {
+  int k = 0;
+  for (int cls = 0; cls <= 3; ++cls)
+    for (int t = 0; t < 27; ++t) {
+      const int ox = t / 9 - 1, oy = (t / 3) % 3 - 1, oz = t % 3 - 1;
+      if (abs(ox) + abs(oy) + abs(oz) != cls) continue;
+      if (k == j) {
+        dx = ox;
+        dy = oy;
+        dz = oz;
+        return cls;
+      }
+      ++k;
+    }
+  return -1;
+}
+
 __host__ __device__ __forceinline__ int wrap_index(int i, int g) {
   if (i < 0) i += g;
   else if (i >= g) i -= g;

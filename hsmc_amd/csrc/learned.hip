@@ -563,6 +563,171 @@ k_voxel_scores_tiled(NGrid g, Vox vx, float far2, double smin, const MlpDev *__r
   score_block(&s, X, H1, ys, red, d, ok, v, f, bmax, bsum);
 }
 
+// Voxels that can hold a free point.  Every point of a voxel lies within
+// half its diagonal h of the voxel's centre, so a voxel whose centre is
+// closer than r = 1 - h to a sphere centre lies inside that sphere's
+// exclusion zone and no insertion in it can succeed.  The proposal gives such
+// voxels no network mass (their score is -inf: q_v = (1 - mix) / V, the
+// uniform part alone, still > 0, so the estimate stays unbiased) and the
+// network scores only the others.
+//
+// Two passes.  k_mark_blocked: one wave per sphere marks (a byte flag)
+// every voxel whose centre's squared distance to it, in FP64 as
+// compute_dist evaluates it (minimum image), is below r^2 -- so a voxel is
+// marked exactly when some sphere blocks it; all marks store the same 1.
+// k_free_tiles: one workgroup per tile of 4 x 8 x 8 voxels sets the score of
+// every marked voxel to -inf and puts the others ("free" voxels) in the
+// tile's own range of `slots` (256 per tile, in thread order) with their
+// number in cnt[tile]; k_gather_free packs them into one list for
+// k_voxel_scores_list.  No global atomics: the tile counts are
+// prefix-summed instead.
+struct Box3 {
+  double lx, ly, lz, hx, hy, hz;
+};
+
+// First candidate voxel index for centres (i + 0.5) * v within r of p: the
+// centres within r have i in the open interval (x, x + 2r/v) with
+// x = (p - r)/v - 0.5, and the 8 indices from floor(x - 1e-9) cover it when
+// 2r/v + 1 < 8 - 1e-6 (the host checks; 6.27 for voxels of edge 0.25), also
+// where rounding puts x on the other side of an integer.  The distance test
+// then decides each candidate.
+__device__ __forceinline__ int voxel_first(double p, double r, double v) {
+  return (int)floor((p - r) / v - 0.5 - 1e-9);
+}
+
+__device__ __forceinline__ int wrap_voxel(int i, int m) {
+  return i < 0 ? i + m : (i >= m ? i - m : i);
+}
+
+__global__ void __launch_bounds__(256)
+k_mark_blocked(const double *__restrict__ x, const double *__restrict__ y,
+               const double *__restrict__ z, int N, Vox vx, Box3 b, double r,
+               unsigned char *__restrict__ flag) {
+  const int lane = threadIdx.x & 63;
+  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (p >= N) return;  // the whole wave
+  const double px = x[p], py = y[p], pz = z[p];
+  const int lx = voxel_first(px, r, vx.vx), ly = voxel_first(py, r, vx.vy),
+            lz = voxel_first(pz, r, vx.vz);
+  const double r2 = r * r;
+  // 8 x 8 x 8 candidates: lane's (y, z) column, 8 x-layers
+  const int iy = wrap_voxel(ly + (lane >> 3), vx.my), iz = wrap_voxel(lz + (lane & 7), vx.mz);
+  const double cy = (iy + 0.5) * vx.vy, cz = (iz + 0.5) * vx.vz;
+#pragma unroll 2
+  for (int k = 0; k < 8; ++k) {
+    const int ix = wrap_voxel(lx + k, vx.mx);
+    const double cx = (ix + 0.5) * vx.vx;
+    if (dist2(cx, cy, cz, px, py, pz, b.lx, b.ly, b.lz, b.hx, b.hy, b.hz) < r2)
+      flag[((long)ix * vx.my + iy) * vx.mz + iz] = 1;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_free_tiles(Vox vx, const unsigned char *__restrict__ flag, float *__restrict__ f,
+             int *__restrict__ slots, int *__restrict__ cnt) {
+  __shared__ int wcount[4];
+  const int ntz = (vx.mz + TZ - 1) / TZ, nty = (vx.my + TY - 1) / TY;
+  const int bz = blockIdx.x % ntz, by = (blockIdx.x / ntz) % nty, bx = blockIdx.x / (ntz * nty);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int ix = bx * TX + w, iy = by * TY + (lane >> 3), iz = bz * TZ + (lane & 7);
+  const bool mine = ix < vx.mx && iy < vx.my && iz < vx.mz;
+  const long v = ((long)ix * vx.my + iy) * vx.mz + iz;
+  bool freev = false;
+  if (mine) {
+    freev = flag[v] == 0;
+    if (!freev) f[v] = -INFINITY;
+  }
+  // the tile's free voxels in thread order: a ballot per wave, then the
+  // waves' counts in order
+  const unsigned long long bal = __ballot(freev);
+  if (lane == 0) wcount[w] = __popcll(bal);
+  __syncthreads();
+  int before = 0;
+  for (int k = 0; k < w; ++k) before += wcount[k];
+  if (freev)
+    slots[(size_t)blockIdx.x * 256 + before + __popcll(bal & ((1ull << lane) - 1ull))] = (int)v;
+  if (threadIdx.x == 0) cnt[blockIdx.x] = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+}
+
+// Pack the tiles' free voxels into one list: tile t's cnt[t] entries go to
+// list[off[t] ...] (off: exclusive prefix sum of cnt).
+__global__ void __launch_bounds__(256)
+k_gather_free(const int *__restrict__ slots, const int *__restrict__ cnt,
+              const int *__restrict__ off, int *__restrict__ list) {
+  const int t = blockIdx.x, k = threadIdx.x;
+  if (k < cnt[t]) list[off[t] + k] = slots[(size_t)t * 256 + k];
+}
+
+// Network scores of the free voxels list[0..n) (descriptor + network, as
+// k_voxel_scores does for every voxel), and per block the max and sum of
+// exp(f - max) over them.
+__global__ void __launch_bounds__(256)
+k_voxel_scores_list(NGrid g, Vox vx, float far2, const MlpDev *__restrict__ net,
+                    const int *__restrict__ list, int n, float *__restrict__ f,
+                    float *__restrict__ bmax, float *__restrict__ bsum) {
+  __shared__ MlpDev s;
+  __shared__ uint16_t X[4][64 * KIN];
+  __shared__ uint16_t H1[4][64 * NH];
+  __shared__ float ys[4][64];
+  __shared__ float red[8];
+  load_net(net, &s);
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const bool ok = i < n;
+  const long v = ok ? (long)list[i] : 0;
+  float d[NIN];
+  if (ok) {
+    double cx, cy, cz;
+    voxel_centre(vx, v, cx, cy, cz);
+    descriptor(g, cx, cy, cz, far2, d);
+  } else {
+    for (int k = 0; k < NIN; ++k) d[k] = 0.f;
+  }
+  score_block(&s, X, H1, ys, red, d, ok, v, f, bmax, bsum);
+}
+
+// Max and sum of exp(f - max) over f[0..V) per block of 4096 voxels, each
+// reduced in a fixed order (strided partials, then a tree), so the totals
+// depend on the scores alone, not on the order in which the free voxels
+// were listed and scored.  Scores of -inf contribute nothing.
+constexpr int kPartVoxels = 4096;
+
+__global__ void __launch_bounds__(256)
+k_score_parts(const float *__restrict__ f, long V, float *__restrict__ bmax,
+              float *__restrict__ bsum) {
+  __shared__ float s[256];
+  const long base = (long)blockIdx.x * kPartVoxels;
+  float m = -INFINITY;
+  for (int k = threadIdx.x; k < kPartVoxels; k += 256) {
+    const long v = base + k;
+    if (v < V) m = fmaxf(m, f[v]);
+  }
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + w]);
+    __syncthreads();
+  }
+  const float bm = s[0];
+  __syncthreads();
+  float e = 0.f;
+  if (bm > -INFINITY) {
+    for (int k = threadIdx.x; k < kPartVoxels; k += 256) {
+      const long v = base + k;
+      if (v < V) e += expf(f[v] - bm);
+    }
+  }
+  s[threadIdx.x] = e;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    bmax[blockIdx.x] = bm;
+    bsum[blockIdx.x] = s[0];
+  }
+}
+
 // Proposal mass of voxel v: q_v = mix softmax(f)_v + (1 - mix) / V.
 __device__ __forceinline__ double proposal_mass(const float *__restrict__ f, long v, long V,
                                                 double fmax, double sumexp, double mix) {
@@ -783,13 +948,15 @@ int grid_cells_for2(double L, double cut) {
   return n;
 }
 
-// scratch slots 24..39 belong to this file (observables.hip uses 0..13,
+// scratch slots 24..42 belong to this file (observables.hip uses 0..13,
 // ctx.hip 21..22, sweep.hip 23): descriptor grid 24..26, overlap grid 27..29,
 // rows / scores 30, targets / block partials 31, cumulative mass 32,
 // sums 34, per-block insertion weights 35, cumulative-mass block sums 36,
-// grid-scan partials 37, voxel scores 38, score block reductions 39.
+// grid-scan partials 37, voxel scores 38, score block reductions 39,
+// free-voxel tile slots and list 40, tile counts, offsets and scan space 41,
+// blocked-voxel flags 42.
 enum { L_GRID_A = 24, L_GRID_B = 27, L_X = 30, L_T = 31, L_CDF = 32, L_ACC = 34, L_PART = 35,
-       L_BSUM = 36, L_SCANP = 37, L_F = 38, L_FRED = 39 };
+       L_BSUM = 36, L_SCANP = 37, L_F = 38, L_FRED = 39, L_LIST = 40, L_NFREE = 41, L_FLAG = 42 };
 
 int build_ngrid(hsmc_gpu_ctx *ctx, int replica, double cut, int slot0, NGrid &g) {
   const RepHost &h = ctx->reps[replica];
@@ -873,6 +1040,91 @@ int launch_scores(hsmc_gpu_ctx *ctx, const NGrid &g, const Vox &vx, const MlpDev
   return HSMC_OK;
 }
 
+// Max and sum of exp(f - max) over the blocks' partials (host side).
+int reduce_partials(hsmc_gpu_ctx *ctx, const float *bred, long nblk, double *fmax, double *sumexp) {
+  std::vector<float> hb(2 * nblk);
+  HSMC_HIP(ctx, hipMemcpyAsync(hb.data(), bred, sizeof(float) * 2 * nblk, hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  double fm = -INFINITY;
+  for (long b = 0; b < nblk; ++b) fm = std::max(fm, (double)hb[b]);
+  double se = 0.0;
+  for (long b = 0; b < nblk; ++b) se += (double)hb[nblk + b] * exp((double)hb[b] - fm);
+  *fmax = fm;
+  *sumexp = se;
+  return HSMC_OK;
+}
+
+// The proposal's scores of one configuration into f: -inf for voxels that
+// cannot hold a free point (k_mark_blocked, k_free_tiles), the network's
+// score for the others (k_voxel_scores_list); *fmax and *sumexp over the
+// scored voxels (0 and 1 when none is free, so every voxel keeps the uniform
+// part alone), *n_free the number scored.  Where a voxel is too large for
+// any exclusion, or too small for the marking kernel's 8 candidates per
+// axis, every voxel is scored.
+int launch_proposal_scores(hsmc_gpu_ctx *ctx, int replica, const NGrid &g, const Vox &vx,
+                           const MlpDev *net, float *f, double *fmax, double *sumexp,
+                           long *n_free) {
+  const double h = 0.5 * sqrt(vx.vx * vx.vx + vx.vy * vx.vy + vx.vz * vx.vz);
+  const double r = 1.0 - h - 1e-9;
+  // the marking kernel's 8 candidate voxels per axis must cover the centres
+  // within r of a sphere (voxel_first), and each axis must have 8 voxels to
+  // wrap over
+  const double lim = 8.0 - 1e-6;
+  const bool span_ok = 2.0 * r / vx.vx + 1.0 < lim && 2.0 * r / vx.vy + 1.0 < lim &&
+                       2.0 * r / vx.vz + 1.0 < lim && vx.mx >= 8 && vx.my >= 8 && vx.mz >= 8;
+  if (!(r > 0.0) || !span_ok) {
+    float *bred = nullptr;
+    long nblk = 0;
+    int st = launch_scores(ctx, g, vx, net, f, &bred, &nblk);
+    if (st) return st;
+    *n_free = vx.V;
+    return reduce_partials(ctx, bred, nblk, fmax, sumexp);
+  }
+  const RepHost &rh = ctx->reps[replica];
+  const long tiles = (long)((vx.mx + TX - 1) / TX) * ((vx.my + TY - 1) / TY) * ((vx.mz + TZ - 1) / TZ);
+  // per tile: 256 free-voxel slots, then the packed list; the tile counts,
+  // their prefix sum and the scan's workspace; a byte per voxel
+  int *slots = (int *)scratch(ctx, L_LIST, sizeof(int) * ((size_t)tiles * 256 + (size_t)vx.V));
+  int *cnt = (int *)scratch(ctx, L_NFREE,
+                            sizeof(int) * (2 * (size_t)tiles + 1 + (size_t)scan_workspace_ints(tiles)));
+  unsigned char *flag = (unsigned char *)scratch(ctx, L_FLAG, (size_t)vx.V);
+  if (!slots || !cnt || !flag) return HSMC_ERR_NO_MEMORY;
+  int *list = slots + (size_t)tiles * 256;
+  int *off = cnt + tiles;
+  int *part = off + tiles + 1;
+  const int N = rh.p.n_part;
+  Box3 b3{rh.p.lx, rh.p.ly, rh.p.lz, rh.p.lx / 2.0, rh.p.ly / 2.0, rh.p.lz / 2.0};
+  HSMC_HIP(ctx, hipMemsetAsync(flag, 0, (size_t)vx.V, ctx->stream));
+  hipLaunchKernelGGL(k_mark_blocked, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, ctx->stream,
+                     ctx->d_x + rh.poff, ctx->d_y + rh.poff, ctx->d_z + rh.poff, N, vx, b3, r, flag);
+  hipLaunchKernelGGL(k_free_tiles, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, vx, flag, f,
+                     slots, cnt);
+  exclusive_scan(ctx->stream, cnt, (int)tiles, off, part);
+  hipLaunchKernelGGL(k_gather_free, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, slots, cnt,
+                     off, list);
+  HSMC_HIP(ctx, hipGetLastError());
+  int nf = 0;
+  HSMC_HIP(ctx, hipMemcpyAsync(&nf, off + tiles, sizeof(nf), hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *n_free = nf;
+  if (nf == 0) {
+    *fmax = 0.0;
+    *sumexp = 1.0;
+    return HSMC_OK;
+  }
+  const long nb = ((long)nf + 255) / 256;
+  const long np = (vx.V + kPartVoxels - 1) / kPartVoxels;
+  float *b = (float *)scratch(ctx, L_FRED, sizeof(float) * 2 * (size_t)std::max(nb, np));
+  if (!b) return HSMC_ERR_NO_MEMORY;
+  // the list's per-block reductions depend on its order; the totals are
+  // taken over the scores in voxel order instead
+  hipLaunchKernelGGL(k_voxel_scores_list, dim3((unsigned)nb), dim3(256), 0, ctx->stream, g, vx,
+                     far2_of(g), net, list, (int)nf, f, b, b + nb);
+  hipLaunchKernelGGL(k_score_parts, dim3((unsigned)np), dim3(256), 0, ctx->stream, f, vx.V, b,
+                     b + np);
+  HSMC_HIP(ctx, hipGetLastError());
+  return reduce_partials(ctx, b, np, fmax, sumexp);
+}
 
 }  // namespace
 
@@ -1068,17 +1320,10 @@ int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mi
                          m->sc_step == m->step && m->sc_voxel == voxel &&
                          m->sc_stamp == ctx->score_stamp;
   if (!scores_ok) {
-    float *bred = nullptr;
-    long nblk = 0;
-    st = launch_scores(ctx, g, vx, m->net, f, &bred, &nblk);
+    double fm = 0.0, se = 0.0;
+    long nfree = 0;
+    st = launch_proposal_scores(ctx, replica, g, vx, m->net, f, &fm, &se, &nfree);
     if (st) return st;
-    std::vector<float> hb(2 * nblk);
-    HSMC_HIP(ctx, hipMemcpyAsync(hb.data(), bred, sizeof(float) * 2 * nblk, hipMemcpyDeviceToHost, ctx->stream));
-    HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    double fm = -INFINITY;
-    for (long b = 0; b < nblk; ++b) fm = std::max(fm, (double)hb[b]);
-    double se = 0.0;
-    for (long b = 0; b < nblk; ++b) se += (double)hb[nblk + b] * exp((double)hb[b] - fm);
     m->sc_replica = replica;
     m->sc_version = ctx->version;
     m->sc_step = m->step;
@@ -1145,6 +1390,34 @@ int hsmc_gpu_voxel_scores(hsmc_gpu_mlp *m, int replica, double voxel, float *sco
   long nblk = 0;
   st = launch_scores(ctx, g, vx, m->net, f, &bred, &nblk);
   if (st) return st;
+  HSMC_HIP(ctx, hipMemcpyAsync(scores, f, sizeof(float) * vx.V, hipMemcpyDeviceToHost, ctx->stream));
+  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HSMC_OK;
+}
+
+int hsmc_gpu_proposal_scores(hsmc_gpu_mlp *m, int replica, double voxel, float *scores,
+                             int64_t cap, int64_t *n_voxels, int64_t *n_free) {
+  if (!m || !(voxel > 0.0) || !n_voxels) return HSMC_ERR_BAD_ARG;
+  hsmc_gpu_ctx *ctx = m->ctx;
+  if (replica < 0 || replica >= ctx->R) return HSMC_ERR_BAD_ARG;
+  const Vox vx = make_vox(ctx->reps[replica], voxel);
+  *n_voxels = vx.V;
+  if (!scores) return HSMC_OK;
+  if (cap < vx.V) return HSMC_ERR_BAD_ARG;
+  if (vx.V > (1L << 31) - 1) return HSMC_ERR_NOT_SUPPORTED;
+  NGrid g;
+  int st = build_ngrid(ctx, replica, kDescCut, L_GRID_A, g);
+  if (st) return st;
+  float *f = (float *)scratch(ctx, L_F, sizeof(float) * (size_t)vx.V);
+  if (!f) return HSMC_ERR_NO_MEMORY;
+  // the shared score buffer is overwritten: no network's cache stays valid
+  ++ctx->score_stamp;
+  m->sc_replica = -1;
+  double fm = 0.0, se = 0.0;
+  long nfree = 0;
+  st = launch_proposal_scores(ctx, replica, g, vx, m->net, f, &fm, &se, &nfree);
+  if (st) return st;
+  if (n_free) *n_free = nfree;
   HSMC_HIP(ctx, hipMemcpyAsync(scores, f, sizeof(float) * vx.V, hipMemcpyDeviceToHost, ctx->stream));
   HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return HSMC_OK;

@@ -271,23 +271,6 @@ k_widom_uniform(Grid g, uint32_t k0, uint32_t k1, unsigned long long sample, lon
 constexpr int kWidomCellsLcap = 256;   // bound of spheres in a 27-cell stencil (host-checked)
 constexpr int kWidomCellsWaves = 4;    // waves per workgroup
 
-// stencil lane order: distance class of offset (dx,dy,dz) is |dx|+|dy|+|dz|
-__device__ __forceinline__ int stencil_by_distance(int j, int &dx, int &dy, int &dz) {
-  // j in [0,27): the 27 offsets sorted by (|dx|+|dy|+|dz|, lexicographic)
-  int k = 0;
-  for (int cls = 0; cls <= 3; ++cls)
-    for (int t = 0; t < 27; ++t) {
-      const int ox = t / 9 - 1, oy = (t / 3) % 3 - 1, oz = t % 3 - 1;
-      if (abs(ox) + abs(oy) + abs(oz) != cls) continue;
-      if (k == j) {
-        dx = ox; dy = oy; dz = oz;
-        return cls;
-      }
-      ++k;
-    }
-  return -1;
-}
-
 __global__ void __launch_bounds__(64 * kWidomCellsWaves)
 k_widom_cells(Grid g, uint32_t k0, uint32_t k1, unsigned long long sample, int per_cell,
               unsigned long long *__restrict__ hits, unsigned *__restrict__ err) {

@@ -37,7 +37,8 @@ EXPORTS = [
     "hsmc_gpu_any_overlap_scaled_noalloc", "hsmc_gpu_widom_cells",
     "hsmc_gpu_mlp_create", "hsmc_gpu_mlp_destroy", "hsmc_gpu_mlp_set", "hsmc_gpu_mlp_get",
     "hsmc_gpu_mlp_forward", "hsmc_gpu_mlp_gradient", "hsmc_gpu_point_descriptors",
-    "hsmc_gpu_mlp_train", "hsmc_gpu_voxel_scores", "hsmc_gpu_widom_learned",
+    "hsmc_gpu_mlp_train", "hsmc_gpu_voxel_scores", "hsmc_gpu_proposal_scores",
+    "hsmc_gpu_widom_learned",
 ]
 
 MLP_IN, MLP_HIDDEN = 12, 64
@@ -138,6 +139,7 @@ def load() -> C.CDLL:
         "hsmc_gpu_mlp_train": (i32, [vp, i32, f64, u64, u64, i64, i32, i32, C.c_float, C.c_float,
                                      P(C.c_float)]),
         "hsmc_gpu_voxel_scores": (i32, [vp, i32, f64, P(C.c_float), i64, P(i64)]),
+        "hsmc_gpu_proposal_scores": (i32, [vp, i32, f64, P(C.c_float), i64, P(i64), P(i64)]),
         "hsmc_gpu_widom_learned": (i32, [vp, i32, f64, f64, u64, u64, i64, P(f64), P(i64),
                                          P(i64)]),
         "hsmc_gpu_occupant_picks": (i32, [i32, u64, C.c_uint32, u64, C.c_uint32, i32,
@@ -450,6 +452,18 @@ class Mlp:
         self.ctx._ck(self.L.hsmc_gpu_voxel_scores(self.h, r, float(voxel), _p(out, C.c_float),
                                                   nv.value, C.byref(nv)), "voxel_scores")
         return out
+
+    def proposal_scores(self, voxel: float, r: int = 0) -> tuple:
+        """The scores the learned proposal uses (-inf where no point of the
+        voxel can be insertable) and the number of voxels scored."""
+        nv, nf = C.c_int64(0), C.c_int64(0)
+        self.ctx._ck(self.L.hsmc_gpu_proposal_scores(self.h, r, float(voxel), None, 0, C.byref(nv),
+                                                     None), "proposal_scores")
+        out = np.zeros(nv.value, dtype=np.float32)
+        self.ctx._ck(self.L.hsmc_gpu_proposal_scores(self.h, r, float(voxel), _p(out, C.c_float),
+                                                     nv.value, C.byref(nv), C.byref(nf)),
+                     "proposal_scores")
+        return out, nf.value
 
     def widom(self, voxel: float, mix: float, seed: int, sample: int, n: int, r: int = 0) -> dict:
         est = C.c_double(0)

@@ -270,14 +270,25 @@ int hsmc_gpu_point_descriptors(hsmc_gpu_ctx *ctx, int replica, const double *xyz
 int hsmc_gpu_mlp_train(hsmc_gpu_mlp *m, int replica, double voxel, uint64_t seed,
                        uint64_t sample, int64_t n_points, int epochs, int batch, float lr,
                        float beta, float *final_loss);
-/* Scores of all voxels (cap >= *n_voxels; scores may be NULL to query V). */
+/* The network's scores of all voxels (cap >= *n_voxels; scores may be NULL
+ * to query V). */
 int hsmc_gpu_voxel_scores(hsmc_gpu_mlp *m, int replica, double voxel, float *scores,
                           int64_t cap, int64_t *n_voxels);
+/* The scores the learned proposal uses: -inf for a voxel whose centre lies
+ * closer than 1 - (half the voxel diagonal) to a sphere centre (no point of
+ * it can be insertable), the network's score for every other voxel (the
+ * free voxels, *n_free of them; n_free may be NULL).  Every voxel is scored
+ * when the voxel is too large for that exclusion (half diagonal >= 1).  No
+ * counterpart in the reference (its insertions are uniform). */
+int hsmc_gpu_proposal_scores(hsmc_gpu_mlp *m, int replica, double voxel, float *scores,
+                             int64_t cap, int64_t *n_voxels, int64_t *n_free);
 /* Importance-sampled Widom: n insertions from q = mix * softmax(score)
- * + (1 - mix) / V, by systematic sampling over voxels (each voxel gets floor
- * or ceil of n q_v, n q_v on average); *estimate = (1/(V n)) sum over
- * insertable points of 1/q_v, an unbiased estimate of the insertion
- * probability; *n_done insertions made (n), *hits insertable. */
+ * + (1 - mix) / V over the proposal scores above (voxels scored -inf keep
+ * the uniform part alone; with no free voxel q is uniform), by systematic
+ * sampling over voxels (each voxel gets floor or ceil of n q_v, n q_v on
+ * average); *estimate = (1/(V n)) sum over insertable points of 1/q_v, an
+ * unbiased estimate of the insertion probability for any q > 0; *n_done
+ * insertions made (n), *hits insertable. */
 int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mix,
                            uint64_t seed, uint64_t sample, int64_t n, double *estimate,
                            int64_t *n_done, int64_t *hits);

@@ -146,6 +146,72 @@ def test_voxel_scores_match_point_path(n, rho):
     ctx.close()
 
 
+def _min_image(d, L):
+    # compute_dist's minimum image (moves.c:419-424), per axis
+    return np.where(d > L / 2.0, d - L, np.where(d < -L / 2.0, d + L, d))
+
+
+@pytest.mark.parametrize("n,rho", [(8, 0.8), (6, 0.6)])
+def test_proposal_excludes_exactly_the_blocked_voxels(n, rho):
+    """The learned proposal scores only voxels that can hold a free point.
+    The voxels it marks -inf are exactly those whose centre lies closer than
+    1 - (half the voxel diagonal) to a sphere centre, with the squared
+    distance in FP64 as compute_dist evaluates it (checked against every
+    sphere within 1 of each centre, found by a periodic k-d tree); the count
+    of free voxels matches; every free voxel's score is the network applied
+    to the per-point descriptor at its centre; and random points inside
+    excluded voxels all overlap a sphere (the exclusion radius is
+    geometrically safe)."""
+    from scipy.spatial import cKDTree
+    ctx, L = _ctx_with(n, rho, 200)
+    net = gpu.Mlp(ctx, seed=4)
+    net.train(0.25, 5, 0, 100000, epochs=2, batch=4096, lr=2e-3, beta=10.0)
+    net.voxel_scores(0.25)   # leaves finite scores in the shared buffer first
+    f, n_free = net.proposal_scores(0.25)
+    m = int(math.ceil(L / 0.25))
+    side = L / m
+    assert len(f) == m ** 3
+    h = 0.5 * math.sqrt(3 * side * side)
+    r2 = (1.0 - h - 1e-9) ** 2
+    conf = ctx.download()[:, 1:4]
+    i = np.arange(m)
+    c1 = (i + 0.5) * side
+    centres = np.stack(np.meshgrid(c1, c1, c1, indexing="ij"), axis=-1).reshape(-1, 3)
+    tree = cKDTree(np.mod(conf, L), boxsize=L)
+    near = tree.query_ball_point(np.mod(centres, L), r=1.0)
+    blocked = np.zeros(len(centres), dtype=bool)
+    for v, js in enumerate(near):
+        if js:
+            d = _min_image(centres[v][None, :] - conf[js], L)
+            d2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+            blocked[v] = bool((d2 < r2).any())
+    assert np.array_equal(np.isneginf(f), blocked)
+    assert n_free == int((~blocked).sum()) and 0 < n_free < len(f)
+    free = np.flatnonzero(~blocked)
+    ref = net.forward(gpu.point_descriptors(ctx, centres[free]))
+    diff = np.abs(f[free] - ref)
+    scale = max(1.0, float(np.max(np.abs(ref))))
+    assert np.quantile(diff, 0.99) <= 1e-4 * scale and diff.max() <= 0.05 * scale
+    rng = np.random.default_rng(3)
+    ex = rng.choice(np.flatnonzero(blocked), size=2000, replace=False)
+    pts = centres[ex] + (rng.random((2000, 3)) - 0.5) * side
+    dmin, _ = cKDTree(np.mod(conf, L), boxsize=L).query(np.mod(pts, L), k=1)
+    assert (dmin < 1.0).all()
+    ctx.close()
+
+
+def test_proposal_scores_every_voxel_when_voxels_are_large():
+    """A voxel of edge 1.3 has a half diagonal above 1: no voxel can be
+    excluded, and every voxel gets the network's score."""
+    ctx, L = _ctx_with(6, 0.6, 50)
+    net = gpu.Mlp(ctx, seed=4)
+    f, n_free = net.proposal_scores(1.3)
+    assert n_free == len(f) and np.isfinite(f).all()
+    raw = net.voxel_scores(1.3)
+    assert np.array_equal(f, raw)
+    ctx.close()
+
+
 def test_two_networks_share_a_context():
     """Two networks on one context alternate learned Widom calls with
     voxel_scores calls, all of which write the context's shared score and
