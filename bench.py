@@ -395,13 +395,15 @@ def _widom_chain(ctx, advance, target, S, gap, n_u, n_l, seed=0, train_points=1_
     (replica 0).  The network is trained on the current configuration first
     (timed, and charged to the learned method).  Then S samples, each `gap`
     chain steps after the last (advance(gap); reported apart and charged to
-    neither method, since both use the same configurations): plain uniform
-    insertion with n_u points, and the learned proposal with n_l points,
-    whose call scores the new configuration's voxels and rebuilds its
-    cumulative mass (charged).  sigma(beta mu_ex) comes from blocking over
-    the per-configuration estimates (hsmc_stat.py:6-50), so configurational
-    variance is sampled; GPU seconds to the target sigma follow the
-    1/samples law from the measured seconds."""
+    neither method, since both use the same configurations).  At each sample
+    every method runs at several insertion counts on the same configuration:
+    uniform (cell-stratified) at about n_u, n_u/4 and n_u/16 points, learned
+    at n_l and n_l/4 insertions, each learned call scoring the configuration
+    again and rebuilding its cumulative mass (charged).  Per count,
+    sigma(beta mu_ex) comes from blocking over the per-configuration
+    estimates (hsmc_stat.py:6-50), so configurational variance is sampled;
+    GPU seconds to the target sigma follow the 1/samples law from the
+    measured seconds.  Each method is reported at its fastest count."""
     import math
 
     import numpy as np
@@ -414,32 +416,42 @@ def _widom_chain(ctx, advance, target, S, gap, n_u, n_l, seed=0, train_points=1_
     loss, t_train = _timed(lambda: net.train(voxel, 777 + seed, 0, train_points, epochs=10,
                                              batch=4096, lr=2e-3, beta=20.0))
     # uniform insertion in its fast, cell-stratified form: per_cell points in
-    # each cell of the insertion grid, about n_u in all
+    # each cell of the insertion grid
     _, _, grid = ctx.widom_cells(0, 0, 0)
     cells = grid[0] * grid[1] * grid[2]
     per_cell = max(1, round(n_u / cells))
-    n_u = per_cell * cells
+    pcs = sorted({max(1, round(per_cell / d)) for d in (1, 4, 16)}, reverse=True)
+    nls = sorted({max(1, n_l // d) for d in (1, 4)}, reverse=True)
     # the i.i.d. uniform kernel (widom_insertion's point stream) on the
     # starting configuration, for its insertion rate
-    _, t_iid = _timed(lambda: ctx.widom_uniform(99, 0, 0, n_u))
-    pu, pl, tu, tl = [], [], [], []
-    hits_u = hits_l = 0
+    _, t_iid = _timed(lambda: ctx.widom_uniform(99, 0, 0, pcs[0] * cells))
+    pu = {pc: [] for pc in pcs}
+    tu = {pc: [] for pc in pcs}
+    hu = {pc: 0 for pc in pcs}
+    pl = {n: [] for n in nls}
+    tl = {n: [] for n in nls}
+    hl = {n: 0 for n in nls}
     t_chain = 0.0
     for k in range(S):
         _, dt = _timed(lambda: (advance(gap), ctx.sync()))
         t_chain += dt
-        (h, _, _), dt = _timed(lambda: ctx.widom_cells(1234 + seed, k, per_cell))
-        pu.append(h / n_u)
-        tu.append(dt)
-        hits_u += h
-        r, dt = _timed(lambda: net.widom(voxel, mix, 4321 + seed, k, n_l))
-        pl.append(r["estimate"])
-        tl.append(dt)
-        hits_l += r["hits"]
+        for pc in pcs:
+            (h, _, _), dt = _timed(lambda: ctx.widom_cells(1234 + seed, k, pc))
+            pu[pc].append(h / (pc * cells))
+            tu[pc].append(dt)
+            hu[pc] += h
+        for n in nls:
+            net.forget()
+            r, dt = _timed(lambda: net.widom(voxel, mix, 4321 + seed, k, n))
+            pl[n].append(r["estimate"])
+            tl[n].append(dt)
+            hl[n] += r["hits"]
     # the variance per insertion on one fixed configuration (the last): 8
-    # repeated estimates per method, so configurational variance is absent
-    rep_u = [ctx.widom_cells(777 + k, 0, per_cell)[0] / n_u for k in range(8)]
-    rep_l = [net.widom(voxel, mix, 888 + k, 0, n_l)["estimate"] for k in range(8)]
+    # repeated estimates per method at its largest count, so configurational
+    # variance is absent
+    n_u0 = pcs[0] * cells
+    rep_u = [ctx.widom_cells(777 + k, 0, pcs[0])[0] / n_u0 for k in range(8)]
+    rep_l = [net.widom(voxel, mix, 888 + k, 0, nls[0])["estimate"] for k in range(8)]
     net.close()
 
     def summary(p, t, n_per, fixed, hits):
@@ -456,14 +468,26 @@ def _widom_chain(ctx, advance, target, S, gap, n_u, n_l, seed=0, train_points=1_
                     blocking_relative=(A.blocking_std(p)[:, 0] / mean).tolist() if mean > 0 else [],
                     ms_per_configuration=1e3 * secs / S, insertions_per_s=n_per * S / secs)
 
-    uni = summary(pu, tu, n_u, 0.0, hits_u)
-    uni["form"] = f"cell-stratified, {per_cell} points in each of {cells} cells"
-    uni["iid_kernel_insertions_per_s"] = n_u / t_iid
-    lrn = summary(pl, tl, n_l, t_train, hits_l)
-    lrn["training_seconds"] = t_train
-    lrn["training_loss"] = loss
-    var_u = float(np.var(rep_u, ddof=1)) * n_u
-    var_l = float(np.var(rep_l, ddof=1)) * n_l
+    def best(table):
+        return min(table, key=lambda d: d["seconds_to_target"])
+
+    unis = []
+    for pc in pcs:
+        d = summary(pu[pc], tu[pc], pc * cells, 0.0, hu[pc])
+        d["form"] = f"cell-stratified, {pc} points in each of {cells} cells"
+        unis.append(d)
+    lrns = []
+    for n in nls:
+        d = summary(pl[n], tl[n], n, t_train, hl[n])
+        d["training_seconds"] = t_train
+        d["training_loss"] = loss
+        lrns.append(d)
+    uni, lrn = best(unis), best(lrns)
+    uni["iid_kernel_insertions_per_s"] = n_u0 / t_iid
+    var_u = float(np.var(rep_u, ddof=1)) * n_u0
+    var_l = float(np.var(rep_l, ddof=1)) * nls[0]
+    brief = lambda d: {k: d[k] for k in ("insertions", "sigma", "ms_per_configuration",
+                                         "seconds_to_target", "seconds_to_target_with_chain")}
     return {
         "ratio": uni["seconds_to_target"] / lrn["seconds_to_target"],
         "ratio_with_chain": uni["seconds_to_target_with_chain"] / lrn["seconds_to_target_with_chain"],
@@ -471,6 +495,7 @@ def _widom_chain(ctx, advance, target, S, gap, n_u, n_l, seed=0, train_points=1_
         "target_sigma": target,
         "insertion_variance_gain_fixed_configuration": var_u / var_l if var_l > 0 else None,
         "uniform": uni, "learned": lrn,
+        "counts_tried": {"uniform": [brief(d) for d in unis], "learned": [brief(d) for d in lrns]},
         "chain": {"gap": gap, "steps": S * gap, "seconds": t_chain},
     }
 
@@ -511,18 +536,18 @@ def widom_headline(args) -> dict:
     ctx, box = _widom_system(0.90, 40000, args.tuning)
     print(f"[bench] widom: N=108000 rho=0.90 melted, box {box[0]:.4f}", file=sys.stderr, flush=True)
     res = _widom_chain(ctx, ctx.sweeps, args.sigma, args.widom_configs, args.widom_gap,
-                       10_000_000, 250_000)
+                       40_000_000, 1_000_000)
     ctx.close()
     print(f"[bench] widom: uniform {res['uniform']}", file=sys.stderr, flush=True)
     print(f"[bench] widom: learned {res['learned']}", file=sys.stderr, flush=True)
     g = int(box[0] // 1.0)
     res["workload"] = ("N=108000 (fcc 4*30^3) rho=0.90 melted by 4*10^4 sweeps; NVT chain, one "
-                       "sample every gap sweeps; uniform (cell-stratified) about 1e7 and learned "
-                       "2.5e5 insertions per sample (the insertion probability of a configuration "
-                       "varies by about 30% from one sample to the next, so larger per-sample "
-                       "budgets buy little); voxel 0.25 sigma (198^3), mix 0.9, network "
-                       "12-64-64-1 trained on 1e6 uniform insertions (training charged to the "
-                       "learned seconds)")
+                       "sample every gap sweeps; each method at each of its insertion counts per "
+                       "sample on the same configurations (uniform, cell-stratified: about 4e7, "
+                       "1e7, 2.5e6 points; learned: 1e6, 2.5e5 insertions, each call scoring its "
+                       "configuration) and reported at its fastest; voxel 0.25 sigma (198^3), mix "
+                       "0.9, network 12-64-64-1 trained on 1e6 uniform insertions (training "
+                       "charged to the learned seconds)")
     res["carnahan_starling_beta_mu_ex"] = A.carnahan_starling(0.90)["mu_ex"]
     res["roofline"] = _widom_kernel_roofline(res, 108000, g ** 3)
     return res

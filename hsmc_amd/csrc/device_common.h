@@ -135,24 +135,38 @@ __host__ __device__ __forceinline__ bool stencil_axis_ok(int d, int g) {
   return d == 0;
 }
 
-// Offset j in [0, 27) of the 27-cell stencil in order of distance class
-// |dx| + |dy| + |dz| (own cell, faces, edges, corners; lexicographic within
-// a class), for scans that can stop at the first sphere found near a point.
-__host__ __device__ __forceinline__ int stencil_by_distance(int j, int &dx, int &dy, int &dz) {
+// The 27-cell stencil in order of distance class |dx| + |dy| + |dz| (own
+// cell, faces, edges, corners; lexicographic within a class), for scans
+// that can stop at the first sphere found near a point.  Entry k of the
+// order is the lexicographic index t = (dx+1)*9 + (dy+1)*3 + (dz+1), kept
+// in 5 bits at position k % 9 of word k / 9; the words are built at compile
+// time, so a lane finds its offset with a few shifts.
+namespace detail {
+constexpr uint64_t stencil_order_word(int w) {
+  uint64_t word = 0;
   int k = 0;
   for (int cls = 0; cls <= 3; ++cls)
     for (int t = 0; t < 27; ++t) {
       const int ox = t / 9 - 1, oy = (t / 3) % 3 - 1, oz = t % 3 - 1;
-      if (abs(ox) + abs(oy) + abs(oz) != cls) continue;
-      if (k == j) {
-        dx = ox;
-        dy = oy;
-        dz = oz;
-        return cls;
-      }
+      const int c = (ox < 0 ? -ox : ox) + (oy < 0 ? -oy : oy) + (oz < 0 ? -oz : oz);
+      if (c != cls) continue;
+      if (k / 9 == w) word |= (uint64_t)t << (5 * (k % 9));
       ++k;
     }
-  return -1;
+  return word;
+}
+}  // namespace detail
+
+// Offset j in [0, 27) of that order; returns its distance class.
+__host__ __device__ __forceinline__ int stencil_by_distance(int j, int &dx, int &dy, int &dz) {
+  constexpr uint64_t w0 = detail::stencil_order_word(0), w1 = detail::stencil_order_word(1),
+                     w2 = detail::stencil_order_word(2);
+  const uint64_t word = j < 9 ? w0 : (j < 18 ? w1 : w2);
+  const int t = (int)((word >> (5 * (j % 9))) & 31u);
+  dx = t / 9 - 1;
+  dy = (t / 3) % 3 - 1;
+  dz = t % 3 - 1;
+  return (dx < 0 ? -dx : dx) + (dy < 0 ? -dy : dy) + (dz < 0 ? -dz : dz);
 }
 
 __host__ __device__ __forceinline__ int wrap_index(int i, int g) {

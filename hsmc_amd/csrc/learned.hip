@@ -1173,6 +1173,12 @@ int hsmc_gpu_mlp_destroy(hsmc_gpu_mlp *m) {
   return HSMC_OK;
 }
 
+int hsmc_gpu_mlp_forget(hsmc_gpu_mlp *m) {
+  if (!m) return HSMC_ERR_BAD_ARG;
+  m->sc_replica = -1;
+  return HSMC_OK;
+}
+
 int hsmc_gpu_mlp_set(hsmc_gpu_mlp *m, const float *params) {
   if (!m || !params) return HSMC_ERR_BAD_ARG;
   hsmc_gpu_ctx *ctx = m->ctx;

@@ -37,7 +37,7 @@ EXPORTS = [
     "hsmc_gpu_any_overlap_scaled_noalloc", "hsmc_gpu_widom_cells",
     "hsmc_gpu_mlp_create", "hsmc_gpu_mlp_destroy", "hsmc_gpu_mlp_set", "hsmc_gpu_mlp_get",
     "hsmc_gpu_mlp_forward", "hsmc_gpu_mlp_gradient", "hsmc_gpu_point_descriptors",
-    "hsmc_gpu_mlp_train", "hsmc_gpu_voxel_scores", "hsmc_gpu_proposal_scores",
+    "hsmc_gpu_mlp_train", "hsmc_gpu_voxel_scores", "hsmc_gpu_proposal_scores", "hsmc_gpu_mlp_forget",
     "hsmc_gpu_widom_learned",
 ]
 
@@ -140,6 +140,7 @@ def load() -> C.CDLL:
                                      P(C.c_float)]),
         "hsmc_gpu_voxel_scores": (i32, [vp, i32, f64, P(C.c_float), i64, P(i64)]),
         "hsmc_gpu_proposal_scores": (i32, [vp, i32, f64, P(C.c_float), i64, P(i64), P(i64)]),
+        "hsmc_gpu_mlp_forget": (i32, [vp]),
         "hsmc_gpu_widom_learned": (i32, [vp, i32, f64, f64, u64, u64, i64, P(f64), P(i64),
                                          P(i64)]),
         "hsmc_gpu_occupant_picks": (i32, [i32, u64, C.c_uint32, u64, C.c_uint32, i32,
@@ -452,6 +453,10 @@ class Mlp:
         self.ctx._ck(self.L.hsmc_gpu_voxel_scores(self.h, r, float(voxel), _p(out, C.c_float),
                                                   nv.value, C.byref(nv)), "voxel_scores")
         return out
+
+    def forget(self) -> None:
+        """Drop the cached proposal, so the next widom call scores again."""
+        self.ctx._ck(self.L.hsmc_gpu_mlp_forget(self.h), "mlp_forget")
 
     def proposal_scores(self, voxel: float, r: int = 0) -> tuple:
         """The scores the learned proposal uses (-inf where no point of the

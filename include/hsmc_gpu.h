@@ -282,6 +282,10 @@ int hsmc_gpu_voxel_scores(hsmc_gpu_mlp *m, int replica, double voxel, float *sco
  * counterpart in the reference (its insertions are uniform). */
 int hsmc_gpu_proposal_scores(hsmc_gpu_mlp *m, int replica, double voxel, float *scores,
                              int64_t cap, int64_t *n_voxels, int64_t *n_free);
+/* Drop the network's cached proposal (scores and cumulative mass) of the
+ * last configuration scored, so that the next learned Widom call scores its
+ * configuration again (for timing each call with its full cost). */
+int hsmc_gpu_mlp_forget(hsmc_gpu_mlp *m);
 /* Importance-sampled Widom: n insertions from q = mix * softmax(score)
  * + (1 - mix) / V over the proposal scores above (voxels scored -inf keep
  * the uniform part alone; with no free voxel q is uniform), by systematic
