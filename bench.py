@@ -593,6 +593,7 @@ def profile_widom(args) -> None:
     for k in range(3):
         ctx.widom_uniform(5, k, 0, 100_000_000)
         ctx.widom_cells(6, k, 850)   # about 1e8 points, cell-stratified
+        ctx.widom_cells(7, k, 85)    # about 1e7 points
     net = gpu.Mlp(ctx, seed=5)
     net.train(0.25, 7, 0, 100_000, epochs=1, batch=4096, lr=2e-3, beta=20.0)
     for k in range(3):
