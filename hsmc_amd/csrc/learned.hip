@@ -575,12 +575,12 @@ k_voxel_scores_tiled(NGrid g, Vox vx, float far2, double smin, const MlpDev *__r
 // every voxel whose centre's squared distance to it, in FP64 as
 // compute_dist evaluates it (minimum image), is below r^2 -- so a voxel is
 // marked exactly when some sphere blocks it; all marks store the same 1.
-// k_free_tiles: one workgroup per tile of 4 x 8 x 8 voxels sets the score of
-// every marked voxel to -inf and puts the others ("free" voxels) in the
-// tile's own range of `slots` (256 per tile, in thread order) with their
-// number in cnt[tile]; k_gather_free packs them into one list for
-// k_voxel_scores_list.  No global atomics: the tile counts are
-// prefix-summed instead.
+// k_free_tiles: one workgroup per tile of 4 x 8 x 8 voxels puts the unmarked
+// ("free") voxels in the tile's own range of `slots` (256 per tile, in
+// thread order) with their number in cnt[tile]; k_gather_free packs them
+// into one list (tile order, then thread order) for k_voxel_scores_list.
+// No global atomics: the tile counts are prefix-summed instead.  Scores are
+// written for free voxels only; the flags tell the others apart.
 struct Box3 {
   double lx, ly, lz, hx, hy, hz;
 };
@@ -623,8 +623,8 @@ k_mark_blocked(const double *__restrict__ x, const double *__restrict__ y,
 }
 
 __global__ void __launch_bounds__(256)
-k_free_tiles(Vox vx, const unsigned char *__restrict__ flag, float *__restrict__ f,
-             int *__restrict__ slots, int *__restrict__ cnt) {
+k_free_tiles(Vox vx, const unsigned char *__restrict__ flag, int *__restrict__ slots,
+             int *__restrict__ cnt) {
   __shared__ int wcount[4];
   const int ntz = (vx.mz + TZ - 1) / TZ, nty = (vx.my + TY - 1) / TY;
   const int bz = blockIdx.x % ntz, by = (blockIdx.x / ntz) % nty, bx = blockIdx.x / (ntz * nty);
@@ -632,11 +632,7 @@ k_free_tiles(Vox vx, const unsigned char *__restrict__ flag, float *__restrict__
   const int ix = bx * TX + w, iy = by * TY + (lane >> 3), iz = bz * TZ + (lane & 7);
   const bool mine = ix < vx.mx && iy < vx.my && iz < vx.mz;
   const long v = ((long)ix * vx.my + iy) * vx.mz + iz;
-  bool freev = false;
-  if (mine) {
-    freev = flag[v] == 0;
-    if (!freev) f[v] = -INFINITY;
-  }
+  const bool freev = mine && flag[v] == 0;
   // the tile's free voxels in thread order: a ballot per wave, then the
   // waves' counts in order
   const unsigned long long bal = __ballot(freev);
@@ -685,21 +681,21 @@ k_voxel_scores_list(NGrid g, Vox vx, float far2, const MlpDev *__restrict__ net,
   score_block(&s, X, H1, ys, red, d, ok, v, f, bmax, bsum);
 }
 
-// Max and sum of exp(f - max) over f[0..V) per block of 4096 voxels, each
-// reduced in a fixed order (strided partials, then a tree), so the totals
-// depend on the scores alone, not on the order in which the free voxels
-// were listed and scored.  Scores of -inf contribute nothing.
-constexpr int kPartVoxels = 4096;
+// Max and sum of exp(f - max) over the scores of list[0..n) per block of
+// 1024 entries, each reduced in a fixed order (strided partials, then a
+// tree), so the totals depend only on the scores and the list's order
+// (fixed by the voxel grid), not on the order in which blocks finished.
+constexpr int kPartVoxels = 1024;
 
 __global__ void __launch_bounds__(256)
-k_score_parts(const float *__restrict__ f, long V, float *__restrict__ bmax,
-              float *__restrict__ bsum) {
+k_score_parts(const float *__restrict__ f, const int *__restrict__ list, long V,
+              float *__restrict__ bmax, float *__restrict__ bsum) {
   __shared__ float s[256];
   const long base = (long)blockIdx.x * kPartVoxels;
   float m = -INFINITY;
   for (int k = threadIdx.x; k < kPartVoxels; k += 256) {
     const long v = base + k;
-    if (v < V) m = fmaxf(m, f[v]);
+    if (v < V) m = fmaxf(m, f[list[v]]);
   }
   s[threadIdx.x] = m;
   __syncthreads();
@@ -713,7 +709,7 @@ k_score_parts(const float *__restrict__ f, long V, float *__restrict__ bmax,
   if (bm > -INFINITY) {
     for (int k = threadIdx.x; k < kPartVoxels; k += 256) {
       const long v = base + k;
-      if (v < V) e += expf(f[v] - bm);
+      if (v < V) e += expf(f[list[v]] - bm);
     }
   }
   s[threadIdx.x] = e;
@@ -728,24 +724,26 @@ k_score_parts(const float *__restrict__ f, long V, float *__restrict__ bmax,
   }
 }
 
-// Proposal mass of voxel v: q_v = mix softmax(f)_v + (1 - mix) / V.
-__device__ __forceinline__ double proposal_mass(const float *__restrict__ f, long v, long V,
-                                                double fmax, double sumexp, double mix) {
-  return mix * exp((double)f[v] - fmax) / sumexp + (1.0 - mix) / (double)V;
+// Network part of the proposal mass of a free voxel v:
+// mix * softmax(f)_v = mix * exp(f_v - fmax) / sumexp.
+__device__ __forceinline__ double network_mass(const float *__restrict__ f, long v, double fmax,
+                                               double sumexp, double mix) {
+  return mix * exp((double)f[v] - fmax) / sumexp;
 }
 
-// Cumulative proposal mass, in three passes: inclusive sums within blocks of
-// 4096 voxels (+ block totals), an exclusive scan of the block totals (one
-// workgroup; its last entry is the total), then the block offsets added.
+// Cumulative network mass over the free-voxel list (list order), in three
+// passes: inclusive sums within blocks of 4096 entries (+ block totals), an
+// exclusive scan of the block totals (one workgroup; its last entry is the
+// total), then the block offsets added.
 __global__ void __launch_bounds__(1024)
-k_cdf_local(const float *__restrict__ f, long V, double fmax, double sumexp, double mix,
-            double *__restrict__ cdf, double *__restrict__ bsum) {
+k_cdf_local(const float *__restrict__ f, const int *__restrict__ list, long V, double fmax,
+            double sumexp, double mix, double *__restrict__ cdf, double *__restrict__ bsum) {
   __shared__ double s[1024];
   const long base = (long)blockIdx.x * 4096;
   double v[4], tot = 0.0;
   for (int j = 0; j < 4; ++j) {
     const long i = base + threadIdx.x * 4 + j;
-    v[j] = i < V ? proposal_mass(f, i, V, fmax, sumexp, mix) : 0.0;
+    v[j] = i < V ? network_mass(f, list[i], fmax, sumexp, mix) : 0.0;
     tot += v[j];
   }
   s[threadIdx.x] = tot;
@@ -815,42 +813,59 @@ __device__ __forceinline__ bool widom_free(const NGrid &g1, double qx, double qy
   return true;
 }
 
-// Systematic sampling of n insertions from the proposal: insertion i takes
-// the voxel whose cumulative mass interval holds (i + u0) / n * total (so a
-// voxel gets floor or ceil of n q_v insertions, n q_v on average), a point
-// uniform inside it, and on a hit the importance weight 1/(V n q_v).
+// Systematic sampling of n insertions from the proposal q_v = (1 - mix) / V
+// + mix softmax(f)_v (the network part on free voxels only).  Insertion i
+// takes t = (i + u0) / n * T, T = (1 - mix) + total (total: the network
+// mass summed over the free list, mix up to rounding).  t below 1 - mix
+// falls in the uniform part, voxel floor(t V / (1 - mix)); above, in the
+// network part: the free voxel whose cumulative-mass interval holds
+// t - (1 - mix).  Each voxel thus covers (1 - mix) / V + its network mass
+// of the t axis and gets floor or ceil of n q_v insertions, n q_v on
+// average.  A point is uniform inside its voxel and, on a hit, weighs
+// 1 / (V n q_v) with q_v its whole mass over T (the flags tell free voxels
+// from blocked ones, whose network mass is 0).
 __global__ void __launch_bounds__(256)
-k_insert(NGrid g1, Vox vx, const double *__restrict__ cdf, double total, long n, double u0,
-         const float *__restrict__ f, double fmax, double sumexp, double mix, uint32_t k0,
-         uint32_t k1, unsigned long long sample, double *__restrict__ part,
+k_insert(NGrid g1, Vox vx, const double *__restrict__ cdf, const int *__restrict__ list,
+         long nlist, double total, long n, double u0, const float *__restrict__ f,
+         const unsigned char *__restrict__ flag, double fmax, double sumexp, double mix,
+         uint32_t k0, uint32_t k1, unsigned long long sample, double *__restrict__ part,
          unsigned long long *__restrict__ hits) {
   __shared__ double rs[4];
   __shared__ unsigned long long rh[4];
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const double un = 1.0 - mix, T = un + total;
   double w = 0.0;
   unsigned long long h = 0;
   if (i < n) {
-    const double t = ((double)i + u0) / (double)n * total;
-    long lo = 0, hi = vx.V - 1;  // first voxel whose cumulative mass exceeds t
-    while (lo < hi) {
-      const long mid = (lo + hi) >> 1;
-      if (cdf[mid] > t)
-        hi = mid;
-      else
-        lo = mid + 1;
+    const double t = ((double)i + u0) / (double)n * T;
+    long v;
+    if (t < un || nlist == 0) {
+      v = (long)(t / un * (double)vx.V);
+      if (v > vx.V - 1) v = vx.V - 1;
+    } else {
+      const double tn = t - un;
+      long lo = 0, hi = nlist - 1;  // first free voxel whose cumulative mass exceeds tn
+      while (lo < hi) {
+        const long mid = (lo + hi) >> 1;
+        if (cdf[mid] > tn)
+          hi = mid;
+        else
+          lo = mid + 1;
+      }
+      v = list[lo];
     }
-    const long v = lo;
     const int iz = (int)(v % vx.mz);
     const long tv = v / vx.mz;
     const int iy = (int)(tv % vx.my), ix = (int)(tv / vx.my);
-    uint32_t u[4];
+  uint32_t u[4];
     philox4x32_10((uint32_t)i, (uint32_t)(i >> 32), (uint32_t)sample,
                   0x4C490000u | (uint32_t)((sample >> 32) & 0xFFFFu), k0, k1, u);
     const double qx = (ix + ((double)u[0] + 0.5) * 0x1p-32) * vx.vx;
     const double qy = (iy + ((double)u[1] + 0.5) * 0x1p-32) * vx.vy;
     const double qz = (iz + ((double)u[2] + 0.5) * 0x1p-32) * vx.vz;
     if (widom_free(g1, qx, qy, qz)) {
-      w = 1.0 / ((double)vx.V * (double)n * (proposal_mass(f, v, vx.V, fmax, sumexp, mix) / total));
+      const double mass = un / (double)vx.V + (flag[v] == 0 ? network_mass(f, v, fmax, sumexp, mix) : 0.0);
+      w = 1.0 / ((double)vx.V * (double)n * (mass / T));
       h = 1;
     }
   }
@@ -906,8 +921,9 @@ struct hsmc_gpu_mlp {
   unsigned long long sc_version = 0;
   long sc_step = -1;
   double sc_voxel = 0.0, sc_fmax = 0.0, sc_sumexp = 0.0;
-  double sc_total = 0.0;  // total of the cached cumulative proposal mass (mix)
+  double sc_total = 0.0;  // total of the cached cumulative network mass (mix)
   double sc_mix = -1.0;
+  long sc_nfree = 0;      // free voxels of the cached proposal
   // the context's score stamp when this network last wrote the shared score
   // and cumulative-mass buffers; another writer (a second network, or
   // voxel_scores) changes the context's stamp and so invalidates the cache
@@ -1054,16 +1070,35 @@ int reduce_partials(hsmc_gpu_ctx *ctx, const float *bred, long nblk, double *fma
   return HSMC_OK;
 }
 
-// The proposal's scores of one configuration into f: -inf for voxels that
-// cannot hold a free point (k_mark_blocked, k_free_tiles), the network's
-// score for the others (k_voxel_scores_list); *fmax and *sumexp over the
-// scored voxels (0 and 1 when none is free, so every voxel keeps the uniform
-// part alone), *n_free the number scored.  Where a voxel is too large for
-// any exclusion, or too small for the marking kernel's 8 candidates per
-// axis, every voxel is scored.
+__global__ void k_iota(int *__restrict__ list, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) list[i] = (int)i;
+}
+
+// -inf for the voxels the flags mark as blocked (the proposal's scores as a
+// caller sees them).
+__global__ void k_mask_blocked(float *__restrict__ f, const unsigned char *__restrict__ flag,
+                               long V) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < V && flag[i]) f[i] = -INFINITY;
+}
+
+// The proposal of one configuration: the byte flags of blocked voxels
+// (k_mark_blocked), the list of free voxels (k_free_tiles, k_gather_free),
+// the network's scores of the free voxels in f (k_voxel_scores_list), and
+// fmax, sumexp over them (0 and 1 when none is free, so every voxel keeps
+// the uniform part alone).  Where a voxel is too large for any exclusion,
+// or too small for the marking kernel's 8 candidates per axis, every voxel
+// is free and scored.
+struct Proposal {
+  unsigned char *flag = nullptr;
+  int *list = nullptr;
+  long nfree = 0;
+  double fmax = 0.0, sumexp = 1.0;
+};
+
 int launch_proposal_scores(hsmc_gpu_ctx *ctx, int replica, const NGrid &g, const Vox &vx,
-                           const MlpDev *net, float *f, double *fmax, double *sumexp,
-                           long *n_free) {
+                           const MlpDev *net, float *f, Proposal *pr) {
   const double h = 0.5 * sqrt(vx.vx * vx.vx + vx.vy * vx.vy + vx.vz * vx.vz);
   const double r = 1.0 - h - 1e-9;
   // the marking kernel's 8 candidate voxels per axis must cover the centres
@@ -1072,15 +1107,6 @@ int launch_proposal_scores(hsmc_gpu_ctx *ctx, int replica, const NGrid &g, const
   const double lim = 8.0 - 1e-6;
   const bool span_ok = 2.0 * r / vx.vx + 1.0 < lim && 2.0 * r / vx.vy + 1.0 < lim &&
                        2.0 * r / vx.vz + 1.0 < lim && vx.mx >= 8 && vx.my >= 8 && vx.mz >= 8;
-  if (!(r > 0.0) || !span_ok) {
-    float *bred = nullptr;
-    long nblk = 0;
-    int st = launch_scores(ctx, g, vx, net, f, &bred, &nblk);
-    if (st) return st;
-    *n_free = vx.V;
-    return reduce_partials(ctx, bred, nblk, fmax, sumexp);
-  }
-  const RepHost &rh = ctx->reps[replica];
   const long tiles = (long)((vx.mx + TX - 1) / TX) * ((vx.my + TY - 1) / TY) * ((vx.mz + TZ - 1) / TZ);
   // per tile: 256 free-voxel slots, then the packed list; the tile counts,
   // their prefix sum and the scan's workspace; a byte per voxel
@@ -1090,14 +1116,28 @@ int launch_proposal_scores(hsmc_gpu_ctx *ctx, int replica, const NGrid &g, const
   unsigned char *flag = (unsigned char *)scratch(ctx, L_FLAG, (size_t)vx.V);
   if (!slots || !cnt || !flag) return HSMC_ERR_NO_MEMORY;
   int *list = slots + (size_t)tiles * 256;
+  pr->flag = flag;
+  pr->list = list;
+  HSMC_HIP(ctx, hipMemsetAsync(flag, 0, (size_t)vx.V, ctx->stream));
+  if (!(r > 0.0) || !span_ok) {
+    float *bred = nullptr;
+    long nblk = 0;
+    int st = launch_scores(ctx, g, vx, net, f, &bred, &nblk);
+    if (st) return st;
+    hipLaunchKernelGGL(k_iota, dim3((unsigned)((vx.V + 255) / 256)), dim3(256), 0, ctx->stream, list,
+                       vx.V);
+    HSMC_HIP(ctx, hipGetLastError());
+    pr->nfree = vx.V;
+    return reduce_partials(ctx, bred, nblk, &pr->fmax, &pr->sumexp);
+  }
+  const RepHost &rh = ctx->reps[replica];
   int *off = cnt + tiles;
   int *part = off + tiles + 1;
   const int N = rh.p.n_part;
   Box3 b3{rh.p.lx, rh.p.ly, rh.p.lz, rh.p.lx / 2.0, rh.p.ly / 2.0, rh.p.lz / 2.0};
-  HSMC_HIP(ctx, hipMemsetAsync(flag, 0, (size_t)vx.V, ctx->stream));
   hipLaunchKernelGGL(k_mark_blocked, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, ctx->stream,
                      ctx->d_x + rh.poff, ctx->d_y + rh.poff, ctx->d_z + rh.poff, N, vx, b3, r, flag);
-  hipLaunchKernelGGL(k_free_tiles, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, vx, flag, f,
+  hipLaunchKernelGGL(k_free_tiles, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, vx, flag,
                      slots, cnt);
   exclusive_scan(ctx->stream, cnt, (int)tiles, off, part);
   hipLaunchKernelGGL(k_gather_free, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, slots, cnt,
@@ -1106,24 +1146,24 @@ int launch_proposal_scores(hsmc_gpu_ctx *ctx, int replica, const NGrid &g, const
   int nf = 0;
   HSMC_HIP(ctx, hipMemcpyAsync(&nf, off + tiles, sizeof(nf), hipMemcpyDeviceToHost, ctx->stream));
   HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  *n_free = nf;
+  pr->nfree = nf;
   if (nf == 0) {
-    *fmax = 0.0;
-    *sumexp = 1.0;
+    pr->fmax = 0.0;
+    pr->sumexp = 1.0;
     return HSMC_OK;
   }
   const long nb = ((long)nf + 255) / 256;
-  const long np = (vx.V + kPartVoxels - 1) / kPartVoxels;
+  const long np = ((long)nf + kPartVoxels - 1) / kPartVoxels;
   float *b = (float *)scratch(ctx, L_FRED, sizeof(float) * 2 * (size_t)std::max(nb, np));
   if (!b) return HSMC_ERR_NO_MEMORY;
-  // the list's per-block reductions depend on its order; the totals are
-  // taken over the scores in voxel order instead
+  // the scoring kernel's per-block reductions are not used: the totals are
+  // taken per 1024 list entries in list order
   hipLaunchKernelGGL(k_voxel_scores_list, dim3((unsigned)nb), dim3(256), 0, ctx->stream, g, vx,
                      far2_of(g), net, list, (int)nf, f, b, b + nb);
-  hipLaunchKernelGGL(k_score_parts, dim3((unsigned)np), dim3(256), 0, ctx->stream, f, vx.V, b,
-                     b + np);
+  hipLaunchKernelGGL(k_score_parts, dim3((unsigned)np), dim3(256), 0, ctx->stream, f, list,
+                     (long)nf, b, b + np);
   HSMC_HIP(ctx, hipGetLastError());
-  return reduce_partials(ctx, b, np, fmax, sumexp);
+  return reduce_partials(ctx, b, np, &pr->fmax, &pr->sumexp);
 }
 
 }  // namespace
@@ -1314,42 +1354,54 @@ int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mi
   if (st) return st;
   const Vox vx = make_vox(ctx->reps[replica], voxel);
   if (vx.V > (1L << 31) - 1) return HSMC_ERR_NOT_SUPPORTED;
-  const long ncdf = (vx.V + 4095) / 4096;
   float *f = (float *)scratch(ctx, L_F, sizeof(float) * (size_t)vx.V);
-  double *cdf = (double *)scratch(ctx, L_CDF, sizeof(double) * (size_t)vx.V);
-  double *bsum = (double *)scratch(ctx, L_BSUM, sizeof(double) * (size_t)(ncdf + 1));
   double *acc = (double *)scratch(ctx, L_ACC, sizeof(double) * 2);
   const long nblk_ins = (long)((n + 255) / 256);
   double *part = (double *)scratch(ctx, L_PART, sizeof(double) * (size_t)nblk_ins);
-  if (!f || !cdf || !bsum || !acc || !part) return HSMC_ERR_NO_MEMORY;
+  if (!f || !acc || !part) return HSMC_ERR_NO_MEMORY;
   const bool scores_ok = m->sc_replica == replica && m->sc_version == ctx->version &&
                          m->sc_step == m->step && m->sc_voxel == voxel &&
                          m->sc_stamp == ctx->score_stamp;
+  Proposal pr;
   if (!scores_ok) {
-    double fm = 0.0, se = 0.0;
-    long nfree = 0;
-    st = launch_proposal_scores(ctx, replica, g, vx, m->net, f, &fm, &se, &nfree);
+    st = launch_proposal_scores(ctx, replica, g, vx, m->net, f, &pr);
     if (st) return st;
     m->sc_replica = replica;
     m->sc_version = ctx->version;
     m->sc_step = m->step;
     m->sc_voxel = voxel;
-    m->sc_fmax = fm;
-    m->sc_sumexp = se;
+    m->sc_fmax = pr.fmax;
+    m->sc_sumexp = pr.sumexp;
+    m->sc_nfree = pr.nfree;
     m->sc_mix = -1.0;  // the cumulative mass is rebuilt below
     m->sc_stamp = ++ctx->score_stamp;
   }
+  // the cached proposal's flags and free list (the same scratch the scoring
+  // wrote; the stamp check above guarantees nobody rewrote it since)
+  const long tiles = (long)((vx.mx + TX - 1) / TX) * ((vx.my + TY - 1) / TY) * ((vx.mz + TZ - 1) / TZ);
+  int *slots = (int *)scratch(ctx, L_LIST, sizeof(int) * ((size_t)tiles * 256 + (size_t)vx.V));
+  unsigned char *flag = (unsigned char *)scratch(ctx, L_FLAG, (size_t)vx.V);
+  if (!slots || !flag) return HSMC_ERR_NO_MEMORY;
+  const int *list = slots + (size_t)tiles * 256;
+  const long nfree = m->sc_nfree;
+  const long ncdf = std::max(1L, (nfree + 4095) / 4096);
+  double *cdf = (double *)scratch(ctx, L_CDF, sizeof(double) * (size_t)std::max(1L, nfree));
+  double *bsum = (double *)scratch(ctx, L_BSUM, sizeof(double) * (size_t)(ncdf + 1));
+  if (!cdf || !bsum) return HSMC_ERR_NO_MEMORY;
   const double fmax = m->sc_fmax, sumexp = m->sc_sumexp;
   if (m->sc_mix != mix) {
-    hipLaunchKernelGGL(k_cdf_local, dim3((unsigned)ncdf), dim3(1024), 0, ctx->stream, f, vx.V, fmax,
-                       sumexp, mix, cdf, bsum);
-    hipLaunchKernelGGL(k_cdf_blocks, dim3(1), dim3(1024), 0, ctx->stream, bsum, (int)ncdf);
-    hipLaunchKernelGGL(k_cdf_add, dim3((unsigned)((vx.V + 255) / 256)), dim3(256), 0, ctx->stream,
-                       cdf, vx.V, bsum);
-    HSMC_HIP(ctx, hipGetLastError());
     double total = 0.0;
-    HSMC_HIP(ctx, hipMemcpyAsync(&total, bsum + ncdf, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (nfree > 0) {
+      hipLaunchKernelGGL(k_cdf_local, dim3((unsigned)ncdf), dim3(1024), 0, ctx->stream, f, list,
+                         nfree, fmax, sumexp, mix, cdf, bsum);
+      hipLaunchKernelGGL(k_cdf_blocks, dim3(1), dim3(1024), 0, ctx->stream, bsum, (int)ncdf);
+      hipLaunchKernelGGL(k_cdf_add, dim3((unsigned)((nfree + 255) / 256)), dim3(256), 0,
+                         ctx->stream, cdf, nfree, bsum);
+      HSMC_HIP(ctx, hipGetLastError());
+      HSMC_HIP(ctx, hipMemcpyAsync(&total, bsum + ncdf, sizeof(double), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+      HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     m->sc_total = total;
     m->sc_mix = mix;
     m->sc_stamp = ++ctx->score_stamp;
@@ -1361,8 +1413,8 @@ int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mi
   const double u0 = (double)u[0] * 0x1p-32;
   HSMC_HIP(ctx, hipMemsetAsync(acc, 0, sizeof(double) * 2, ctx->stream));
   hipLaunchKernelGGL(k_insert, dim3((unsigned)nblk_ins), dim3(256), 0, ctx->stream, g1, vx,
-                     cdf, m->sc_total, (long)n, u0, f, fmax, sumexp, mix, (uint32_t)seed,
-                     (uint32_t)(seed >> 32), (unsigned long long)sample, part,
+                     cdf, list, nfree, m->sc_total, (long)n, u0, f, flag, fmax, sumexp, mix,
+                     (uint32_t)seed, (uint32_t)(seed >> 32), (unsigned long long)sample, part,
                      reinterpret_cast<unsigned long long *>(acc + 1));
   hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(256), 0, ctx->stream, part, nblk_ins, acc);
   HSMC_HIP(ctx, hipGetLastError());
@@ -1419,11 +1471,13 @@ int hsmc_gpu_proposal_scores(hsmc_gpu_mlp *m, int replica, double voxel, float *
   // the shared score buffer is overwritten: no network's cache stays valid
   ++ctx->score_stamp;
   m->sc_replica = -1;
-  double fm = 0.0, se = 0.0;
-  long nfree = 0;
-  st = launch_proposal_scores(ctx, replica, g, vx, m->net, f, &fm, &se, &nfree);
+  Proposal pr;
+  st = launch_proposal_scores(ctx, replica, g, vx, m->net, f, &pr);
   if (st) return st;
-  if (n_free) *n_free = nfree;
+  hipLaunchKernelGGL(k_mask_blocked, dim3((unsigned)((vx.V + 255) / 256)), dim3(256), 0,
+                     ctx->stream, f, pr.flag, vx.V);
+  HSMC_HIP(ctx, hipGetLastError());
+  if (n_free) *n_free = pr.nfree;
   HSMC_HIP(ctx, hipMemcpyAsync(scores, f, sizeof(float) * vx.V, hipMemcpyDeviceToHost, ctx->stream));
   HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return HSMC_OK;
