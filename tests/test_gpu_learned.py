@@ -151,8 +151,18 @@ def _min_image(d, L):
     return np.where(d > L / 2.0, d - L, np.where(d < -L / 2.0, d + L, d))
 
 
-@pytest.mark.parametrize("n,rho", [(8, 0.8), (6, 0.6)])
-def test_proposal_excludes_exactly_the_blocked_voxels(n, rho):
+def _anisotropic_ctx():
+    """fcc 8 x 6 x 7 cells at rho = 0.8 (three different box edges), melted
+    by 300 sweeps on the GPU."""
+    lat, box = gpu.lattice(2, (8, 6, 7), 0.8)
+    ctx = gpu.Context([dict(n=len(lat), box=box, dr_max=0.12, seed=7)])
+    ctx.upload(lat)
+    ctx.sweeps(300)
+    return ctx, tuple(float(b) for b in box)
+
+
+@pytest.mark.parametrize("case", ["cube-0.8", "cube-0.6", "anisotropic-0.8"])
+def test_proposal_excludes_exactly_the_blocked_voxels(case):
     """The learned proposal scores only voxels that can hold a free point.
     The voxels it marks -inf are exactly those whose centre lies closer than
     1 - (half the voxel diagonal) to a sphere centre, with the squared
@@ -161,28 +171,34 @@ def test_proposal_excludes_exactly_the_blocked_voxels(n, rho):
     of free voxels matches; every free voxel's score is the network applied
     to the per-point descriptor at its centre; and random points inside
     excluded voxels all overlap a sphere (the exclusion radius is
-    geometrically safe)."""
+    geometrically safe).  Cubic boxes at rho = 0.8 and 0.6, and a box with
+    three different edges (so three voxel counts and edges)."""
     from scipy.spatial import cKDTree
-    ctx, L = _ctx_with(n, rho, 200)
+    if case == "anisotropic-0.8":
+        ctx, box = _anisotropic_ctx()
+    else:
+        n, rho = {"cube-0.8": (8, 0.8), "cube-0.6": (6, 0.6)}[case]
+        ctx, L = _ctx_with(n, rho, 200)
+        box = (L, L, L)
     net = gpu.Mlp(ctx, seed=4)
     net.train(0.25, 5, 0, 100000, epochs=2, batch=4096, lr=2e-3, beta=10.0)
     net.voxel_scores(0.25)   # leaves finite scores in the shared buffer first
     f, n_free = net.proposal_scores(0.25)
-    m = int(math.ceil(L / 0.25))
-    side = L / m
-    assert len(f) == m ** 3
-    h = 0.5 * math.sqrt(3 * side * side)
+    m = [int(math.ceil(b / 0.25)) for b in box]
+    side = [b / k for b, k in zip(box, m)]
+    assert len(f) == m[0] * m[1] * m[2]
+    h = 0.5 * math.sqrt(side[0] * side[0] + side[1] * side[1] + side[2] * side[2])
     r2 = (1.0 - h - 1e-9) ** 2
     conf = ctx.download()[:, 1:4]
-    i = np.arange(m)
-    c1 = (i + 0.5) * side
-    centres = np.stack(np.meshgrid(c1, c1, c1, indexing="ij"), axis=-1).reshape(-1, 3)
-    tree = cKDTree(np.mod(conf, L), boxsize=L)
-    near = tree.query_ball_point(np.mod(centres, L), r=1.0)
+    axes = [(np.arange(k) + 0.5) * sd for k, sd in zip(m, side)]
+    centres = np.stack(np.meshgrid(*axes, indexing="ij"), axis=-1).reshape(-1, 3)
+    Lv = np.array(box)
+    tree = cKDTree(np.mod(conf, Lv), boxsize=Lv)
+    near = tree.query_ball_point(np.mod(centres, Lv), r=1.0)
     blocked = np.zeros(len(centres), dtype=bool)
     for v, js in enumerate(near):
         if js:
-            d = _min_image(centres[v][None, :] - conf[js], L)
+            d = np.stack([_min_image(centres[v][k] - conf[js, k], box[k]) for k in range(3)], axis=1)
             d2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
             blocked[v] = bool((d2 < r2).any())
     assert np.array_equal(np.isneginf(f), blocked)
@@ -194,8 +210,8 @@ def test_proposal_excludes_exactly_the_blocked_voxels(n, rho):
     assert np.quantile(diff, 0.99) <= 1e-4 * scale and diff.max() <= 0.05 * scale
     rng = np.random.default_rng(3)
     ex = rng.choice(np.flatnonzero(blocked), size=2000, replace=False)
-    pts = centres[ex] + (rng.random((2000, 3)) - 0.5) * side
-    dmin, _ = cKDTree(np.mod(conf, L), boxsize=L).query(np.mod(pts, L), k=1)
+    pts = centres[ex] + (rng.random((2000, 3)) - 0.5) * np.array(side)
+    dmin, _ = tree.query(np.mod(pts, Lv), k=1)
     assert (dmin < 1.0).all()
     ctx.close()
 
