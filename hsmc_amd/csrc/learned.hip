@@ -725,10 +725,53 @@ k_score_parts(const float *__restrict__ f, const int *__restrict__ list, long V,
 }
 
 // Network part of the proposal mass of a free voxel v:
-// mix * softmax(f)_v = mix * exp(f_v - fmax) / sumexp.
-__device__ __forceinline__ double network_mass(const float *__restrict__ f, long v, double fmax,
-                                               double sumexp, double mix) {
-  return mix * exp((double)f[v] - fmax) / sumexp;
+// mix * softmax(f)_v = mix * exp(f_v - fmax) / sumexp, with st[0] = fmax and
+// st[1] = sumexp (on the device, written by k_reduce_parts).
+__device__ __forceinline__ double network_mass(const float *__restrict__ f, long v,
+                                               const double *__restrict__ st, double mix) {
+  return mix * exp((double)f[v] - st[0]) / st[1];
+}
+
+// st[0] = max of the blocks' maxima, st[1] = sum of their sums of
+// exp(f - max) rescaled to st[0], each in a fixed order (strided partials,
+// then a tree); st[2] = 0.
+__global__ void __launch_bounds__(256)
+k_reduce_parts(const float *__restrict__ bmax, const float *__restrict__ bsum, long nb,
+               double *__restrict__ st) {
+  __shared__ double s[256];
+  double m = -INFINITY;
+  for (long b = threadIdx.x; b < nb; b += 256) m = fmax(m, (double)bmax[b]);
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+    __syncthreads();
+  }
+  const double fm = s[0];
+  __syncthreads();
+  double e = 0.0;
+  for (long b = threadIdx.x; b < nb; b += 256) e += (double)bsum[b] * exp((double)bmax[b] - fm);
+  s[threadIdx.x] = e;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    st[0] = fm;
+    st[1] = s[0];
+    st[2] = 0.0;
+  }
+}
+
+// The statistics of a proposal with no free voxel (every voxel keeps the
+// uniform part alone).
+__global__ void k_empty_stats(double *__restrict__ st) {
+  if (threadIdx.x == 0) {
+    st[0] = 0.0;
+    st[1] = 1.0;
+    st[2] = 0.0;
+  }
 }
 
 // Cumulative network mass over the free-voxel list (list order), in three
@@ -736,14 +779,15 @@ __device__ __forceinline__ double network_mass(const float *__restrict__ f, long
 // exclusive scan of the block totals (one workgroup; its last entry is the
 // total), then the block offsets added.
 __global__ void __launch_bounds__(1024)
-k_cdf_local(const float *__restrict__ f, const int *__restrict__ list, long V, double fmax,
-            double sumexp, double mix, double *__restrict__ cdf, double *__restrict__ bsum) {
+k_cdf_local(const float *__restrict__ f, const int *__restrict__ list, long V,
+            const double *__restrict__ st, double mix, double *__restrict__ cdf,
+            double *__restrict__ bsum) {
   __shared__ double s[1024];
   const long base = (long)blockIdx.x * 4096;
   double v[4], tot = 0.0;
   for (int j = 0; j < 4; ++j) {
     const long i = base + threadIdx.x * 4 + j;
-    v[j] = i < V ? network_mass(f, list[i], fmax, sumexp, mix) : 0.0;
+    v[j] = i < V ? network_mass(f, list[i], st, mix) : 0.0;
     tot += v[j];
   }
   s[threadIdx.x] = tot;
@@ -815,8 +859,8 @@ __device__ __forceinline__ bool widom_free(const NGrid &g1, double qx, double qy
 
 // Systematic sampling of n insertions from the proposal q_v = (1 - mix) / V
 // + mix softmax(f)_v (the network part on free voxels only).  Insertion i
-// takes t = (i + u0) / n * T, T = (1 - mix) + total (total: the network
-// mass summed over the free list, mix up to rounding).  t below 1 - mix
+// takes t = (i + u0) / n * T, T = (1 - mix) + *total (the network mass
+// summed over the free list, mix up to rounding; 0 with no free voxel).  t below 1 - mix
 // falls in the uniform part, voxel floor(t V / (1 - mix)); above, in the
 // network part: the free voxel whose cumulative-mass interval holds
 // t - (1 - mix).  Each voxel thus covers (1 - mix) / V + its network mass
@@ -826,14 +870,15 @@ __device__ __forceinline__ bool widom_free(const NGrid &g1, double qx, double qy
 // from blocked ones, whose network mass is 0).
 __global__ void __launch_bounds__(256)
 k_insert(NGrid g1, Vox vx, const double *__restrict__ cdf, const int *__restrict__ list,
-         long nlist, double total, long n, double u0, const float *__restrict__ f,
-         const unsigned char *__restrict__ flag, double fmax, double sumexp, double mix,
-         uint32_t k0, uint32_t k1, unsigned long long sample, double *__restrict__ part,
+         long nlist, const double *__restrict__ total, long n, double u0,
+         const float *__restrict__ f, const unsigned char *__restrict__ flag,
+         const double *__restrict__ st, double mix, uint32_t k0, uint32_t k1,
+         unsigned long long sample, double *__restrict__ part,
          unsigned long long *__restrict__ hits) {
   __shared__ double rs[4];
   __shared__ unsigned long long rh[4];
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const double un = 1.0 - mix, T = un + total;
+  const double un = 1.0 - mix, T = un + *total;
   double w = 0.0;
   unsigned long long h = 0;
   if (i < n) {
@@ -864,7 +909,7 @@ k_insert(NGrid g1, Vox vx, const double *__restrict__ cdf, const int *__restrict
     const double qy = (iy + ((double)u[1] + 0.5) * 0x1p-32) * vx.vy;
     const double qz = (iz + ((double)u[2] + 0.5) * 0x1p-32) * vx.vz;
     if (widom_free(g1, qx, qy, qz)) {
-      const double mass = un / (double)vx.V + (flag[v] == 0 ? network_mass(f, v, fmax, sumexp, mix) : 0.0);
+      const double mass = un / (double)vx.V + (flag[v] == 0 ? network_mass(f, v, st, mix) : 0.0);
       w = 1.0 / ((double)vx.V * (double)n * (mass / T));
       h = 1;
     }
@@ -915,14 +960,14 @@ struct hsmc_gpu_mlp {
   MlpDev *net = nullptr;
   float *loss = nullptr;
   long step = 0;
-  // voxel scores of the last configuration scored (replica, version, step,
-  // voxel) with their max and sum exp(f - max)
+  // the proposal of the last configuration scored (replica, version, step,
+  // voxel); its scores, flags, free list and softmax statistics stay in the
+  // context's scratch
   int sc_replica = -1;
   unsigned long long sc_version = 0;
   long sc_step = -1;
-  double sc_voxel = 0.0, sc_fmax = 0.0, sc_sumexp = 0.0;
-  double sc_total = 0.0;  // total of the cached cumulative network mass (mix)
-  double sc_mix = -1.0;
+  double sc_voxel = 0.0;
+  double sc_mix = -1.0;   // mix of the cached cumulative network mass
   long sc_nfree = 0;      // free voxels of the cached proposal
   // the context's score stamp when this network last wrote the shared score
   // and cumulative-mass buffers; another writer (a second network, or
@@ -964,15 +1009,16 @@ int grid_cells_for2(double L, double cut) {
   return n;
 }
 
-// scratch slots 24..42 belong to this file (observables.hip uses 0..13,
+// scratch slots 24..43 belong to this file (observables.hip uses 0..13,
 // ctx.hip 21..22, sweep.hip 23): descriptor grid 24..26, overlap grid 27..29,
 // rows / scores 30, targets / block partials 31, cumulative mass 32,
 // sums 34, per-block insertion weights 35, cumulative-mass block sums 36,
 // grid-scan partials 37, voxel scores 38, score block reductions 39,
 // free-voxel tile slots and list 40, tile counts, offsets and scan space 41,
-// blocked-voxel flags 42.
+// blocked-voxel flags 42, softmax statistics 43.
 enum { L_GRID_A = 24, L_GRID_B = 27, L_X = 30, L_T = 31, L_CDF = 32, L_ACC = 34, L_PART = 35,
-       L_BSUM = 36, L_SCANP = 37, L_F = 38, L_FRED = 39, L_LIST = 40, L_NFREE = 41, L_FLAG = 42 };
+       L_BSUM = 36, L_SCANP = 37, L_F = 38, L_FRED = 39, L_LIST = 40, L_NFREE = 41, L_FLAG = 42,
+       L_STATS = 43 };
 
 int build_ngrid(hsmc_gpu_ctx *ctx, int replica, double cut, int slot0, NGrid &g) {
   const RepHost &h = ctx->reps[replica];
@@ -1056,20 +1102,6 @@ int launch_scores(hsmc_gpu_ctx *ctx, const NGrid &g, const Vox &vx, const MlpDev
   return HSMC_OK;
 }
 
-// Max and sum of exp(f - max) over the blocks' partials (host side).
-int reduce_partials(hsmc_gpu_ctx *ctx, const float *bred, long nblk, double *fmax, double *sumexp) {
-  std::vector<float> hb(2 * nblk);
-  HSMC_HIP(ctx, hipMemcpyAsync(hb.data(), bred, sizeof(float) * 2 * nblk, hipMemcpyDeviceToHost, ctx->stream));
-  HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  double fm = -INFINITY;
-  for (long b = 0; b < nblk; ++b) fm = std::max(fm, (double)hb[b]);
-  double se = 0.0;
-  for (long b = 0; b < nblk; ++b) se += (double)hb[nblk + b] * exp((double)hb[b] - fm);
-  *fmax = fm;
-  *sumexp = se;
-  return HSMC_OK;
-}
-
 __global__ void k_iota(int *__restrict__ list, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) list[i] = (int)i;
@@ -1086,15 +1118,16 @@ __global__ void k_mask_blocked(float *__restrict__ f, const unsigned char *__res
 // The proposal of one configuration: the byte flags of blocked voxels
 // (k_mark_blocked), the list of free voxels (k_free_tiles, k_gather_free),
 // the network's scores of the free voxels in f (k_voxel_scores_list), and
-// fmax, sumexp over them (0 and 1 when none is free, so every voxel keeps
-// the uniform part alone).  Where a voxel is too large for any exclusion,
-// or too small for the marking kernel's 8 candidates per axis, every voxel
-// is free and scored.
+// in stats[0], stats[1] (device) fmax and sumexp over them (0 and 1 when none is
+// free, so every voxel keeps the uniform part alone).  Where a voxel is too
+// large for any exclusion, or too small for the marking kernel's 8
+// candidates per axis, every voxel is free and scored.  Only the number of
+// free voxels comes back to the host (it sizes the later launches).
 struct Proposal {
   unsigned char *flag = nullptr;
   int *list = nullptr;
+  double *stats = nullptr;
   long nfree = 0;
-  double fmax = 0.0, sumexp = 1.0;
 };
 
 int launch_proposal_scores(hsmc_gpu_ctx *ctx, int replica, const NGrid &g, const Vox &vx,
@@ -1114,10 +1147,12 @@ int launch_proposal_scores(hsmc_gpu_ctx *ctx, int replica, const NGrid &g, const
   int *cnt = (int *)scratch(ctx, L_NFREE,
                             sizeof(int) * (2 * (size_t)tiles + 1 + (size_t)scan_workspace_ints(tiles)));
   unsigned char *flag = (unsigned char *)scratch(ctx, L_FLAG, (size_t)vx.V);
-  if (!slots || !cnt || !flag) return HSMC_ERR_NO_MEMORY;
+  double *stats = (double *)scratch(ctx, L_STATS, sizeof(double) * 3);
+  if (!slots || !cnt || !flag || !stats) return HSMC_ERR_NO_MEMORY;
   int *list = slots + (size_t)tiles * 256;
   pr->flag = flag;
   pr->list = list;
+  pr->stats = stats;
   HSMC_HIP(ctx, hipMemsetAsync(flag, 0, (size_t)vx.V, ctx->stream));
   if (!(r > 0.0) || !span_ok) {
     float *bred = nullptr;
@@ -1126,9 +1161,11 @@ int launch_proposal_scores(hsmc_gpu_ctx *ctx, int replica, const NGrid &g, const
     if (st) return st;
     hipLaunchKernelGGL(k_iota, dim3((unsigned)((vx.V + 255) / 256)), dim3(256), 0, ctx->stream, list,
                        vx.V);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, ctx->stream, bred, bred + nblk, nblk,
+                       stats);
     HSMC_HIP(ctx, hipGetLastError());
     pr->nfree = vx.V;
-    return reduce_partials(ctx, bred, nblk, &pr->fmax, &pr->sumexp);
+    return HSMC_OK;
   }
   const RepHost &rh = ctx->reps[replica];
   int *off = cnt + tiles;
@@ -1148,8 +1185,8 @@ int launch_proposal_scores(hsmc_gpu_ctx *ctx, int replica, const NGrid &g, const
   HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   pr->nfree = nf;
   if (nf == 0) {
-    pr->fmax = 0.0;
-    pr->sumexp = 1.0;
+    hipLaunchKernelGGL(k_empty_stats, dim3(1), dim3(64), 0, ctx->stream, stats);
+    HSMC_HIP(ctx, hipGetLastError());
     return HSMC_OK;
   }
   const long nb = ((long)nf + 255) / 256;
@@ -1162,8 +1199,9 @@ int launch_proposal_scores(hsmc_gpu_ctx *ctx, int replica, const NGrid &g, const
                      far2_of(g), net, list, (int)nf, f, b, b + nb);
   hipLaunchKernelGGL(k_score_parts, dim3((unsigned)np), dim3(256), 0, ctx->stream, f, list,
                      (long)nf, b, b + np);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, ctx->stream, b, b + np, np, stats);
   HSMC_HIP(ctx, hipGetLastError());
-  return reduce_partials(ctx, b, np, &pr->fmax, &pr->sumexp);
+  return HSMC_OK;
 }
 
 }  // namespace
@@ -1370,8 +1408,6 @@ int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mi
     m->sc_version = ctx->version;
     m->sc_step = m->step;
     m->sc_voxel = voxel;
-    m->sc_fmax = pr.fmax;
-    m->sc_sumexp = pr.sumexp;
     m->sc_nfree = pr.nfree;
     m->sc_mix = -1.0;  // the cumulative mass is rebuilt below
     m->sc_stamp = ++ctx->score_stamp;
@@ -1381,31 +1417,28 @@ int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mi
   const long tiles = (long)((vx.mx + TX - 1) / TX) * ((vx.my + TY - 1) / TY) * ((vx.mz + TZ - 1) / TZ);
   int *slots = (int *)scratch(ctx, L_LIST, sizeof(int) * ((size_t)tiles * 256 + (size_t)vx.V));
   unsigned char *flag = (unsigned char *)scratch(ctx, L_FLAG, (size_t)vx.V);
-  if (!slots || !flag) return HSMC_ERR_NO_MEMORY;
+  double *stats = (double *)scratch(ctx, L_STATS, sizeof(double) * 3);
+  if (!slots || !flag || !stats) return HSMC_ERR_NO_MEMORY;
   const int *list = slots + (size_t)tiles * 256;
   const long nfree = m->sc_nfree;
   const long ncdf = std::max(1L, (nfree + 4095) / 4096);
   double *cdf = (double *)scratch(ctx, L_CDF, sizeof(double) * (size_t)std::max(1L, nfree));
   double *bsum = (double *)scratch(ctx, L_BSUM, sizeof(double) * (size_t)(ncdf + 1));
   if (!cdf || !bsum) return HSMC_ERR_NO_MEMORY;
-  const double fmax = m->sc_fmax, sumexp = m->sc_sumexp;
+  if (m->sc_mix != mix && nfree > 0) {
+    hipLaunchKernelGGL(k_cdf_local, dim3((unsigned)ncdf), dim3(1024), 0, ctx->stream, f, list,
+                       nfree, stats, mix, cdf, bsum);
+    hipLaunchKernelGGL(k_cdf_blocks, dim3(1), dim3(1024), 0, ctx->stream, bsum, (int)ncdf);
+    hipLaunchKernelGGL(k_cdf_add, dim3((unsigned)((nfree + 255) / 256)), dim3(256), 0, ctx->stream,
+                       cdf, nfree, bsum);
+    HSMC_HIP(ctx, hipGetLastError());
+  }
   if (m->sc_mix != mix) {
-    double total = 0.0;
-    if (nfree > 0) {
-      hipLaunchKernelGGL(k_cdf_local, dim3((unsigned)ncdf), dim3(1024), 0, ctx->stream, f, list,
-                         nfree, fmax, sumexp, mix, cdf, bsum);
-      hipLaunchKernelGGL(k_cdf_blocks, dim3(1), dim3(1024), 0, ctx->stream, bsum, (int)ncdf);
-      hipLaunchKernelGGL(k_cdf_add, dim3((unsigned)((nfree + 255) / 256)), dim3(256), 0,
-                         ctx->stream, cdf, nfree, bsum);
-      HSMC_HIP(ctx, hipGetLastError());
-      HSMC_HIP(ctx, hipMemcpyAsync(&total, bsum + ncdf, sizeof(double), hipMemcpyDeviceToHost,
-                                   ctx->stream));
-      HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    }
-    m->sc_total = total;
     m->sc_mix = mix;
     m->sc_stamp = ++ctx->score_stamp;
   }
+  // the network mass's total: the scan's last entry, or 0 with no free voxel
+  const double *total = nfree > 0 ? bsum + ncdf : stats + 2;
   // one uniform offset per call for the systematic sample
   uint32_t u[4];
   philox4x32_10(0u, 0u, (uint32_t)sample, 0x4C530000u | (uint32_t)((sample >> 32) & 0xFFFFu),
@@ -1413,7 +1446,7 @@ int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mi
   const double u0 = (double)u[0] * 0x1p-32;
   HSMC_HIP(ctx, hipMemsetAsync(acc, 0, sizeof(double) * 2, ctx->stream));
   hipLaunchKernelGGL(k_insert, dim3((unsigned)nblk_ins), dim3(256), 0, ctx->stream, g1, vx,
-                     cdf, list, nfree, m->sc_total, (long)n, u0, f, flag, fmax, sumexp, mix,
+                     cdf, list, nfree, total, (long)n, u0, f, flag, stats, mix,
                      (uint32_t)seed, (uint32_t)(seed >> 32), (unsigned long long)sample, part,
                      reinterpret_cast<unsigned long long *>(acc + 1));
   hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(256), 0, ctx->stream, part, nblk_ins, acc);
