@@ -567,9 +567,9 @@ k_voxel_scores_tiled(NGrid g, Vox vx, float far2, double smin, const MlpDev *__r
 // half its diagonal h of the voxel's centre, so a voxel whose centre is
 // closer than r = 1 - h to a sphere centre lies inside that sphere's
 // exclusion zone and no insertion in it can succeed.  The proposal gives such
-// voxels no network mass (their score is -inf: q_v = (1 - mix) / V, the
-// uniform part alone, still > 0, so the estimate stays unbiased) and the
-// network scores only the others.
+// voxels no network mass (q_v = (1 - mix) / V, the uniform part alone, still
+// > 0, so the estimate stays unbiased; their score reads -inf in
+// hsmc_gpu_proposal_scores) and the network scores only the others.
 //
 // Two passes.  k_mark_blocked: one wave per sphere marks (a byte flag)
 // every voxel whose centre's squared distance to it, in FP64 as
