@@ -150,8 +150,8 @@ using dvec = double __attribute__((ext_vector_type(M)));
 template <int CAP, int LPC, int LCAP, bool ONE>
 __device__ __forceinline__ void phase_cell(
     int a, int a0, int lane, int *Lm, const RepDev *__restrict__ reps, const RepDev &rp0,
-    const SweepDev *__restrict__ swp, const int *__restrict__ arep, int A, int phase,
-    unsigned long long sweep, double *__restrict__ x, double *__restrict__ y,
+    const SweepDev *__restrict__ swp, const SweepDev &sw0, const int *__restrict__ arep, int A,
+    int phase, unsigned long long sweep, double *__restrict__ x, double *__restrict__ y,
     double *__restrict__ z, const int *__restrict__ cnt, int C, SlotRec *__restrict__ slot,
     unsigned int *__restrict__ cacc, unsigned int *__restrict__ cleft,
     unsigned int *__restrict__ err) {
@@ -166,23 +166,27 @@ __device__ __forceinline__ void phase_cell(
   double *Op = reinterpret_cast<double *>(Lo + CAP);   // occupants' positions (x, y, z)
   double *Od = Op + 3 * CAP;                           // each attempt's displacement
 
-  if (a >= A) return;  // the whole lane group leaves together
   const int *cntb = cnt + (size_t)(sweep & 1ull) * C;
   // the replica's record (a kernel argument when there is one replica) and
-  // the sweep's shift and sublattice order, which the rebin drew.  Each
-  // replica's active cells are padded to a multiple of 8 and cells_per_block
-  // divides 8, so a workgroup lies in one replica: r is uniform and the
-  // record and table are scalar loads.
+  // the sweep's shift and sublattice order, which the rebin drew (loaded by
+  // the caller when there is one replica).  Each replica's active cells are
+  // padded to a multiple of 8 and cells_per_block divides 8, so a workgroup
+  // lies in one replica: r is uniform and the record and table are scalar
+  // loads (a0 < A always).
   const int r = ONE ? 0 : arep[a0];
   const RepDev rp = ONE ? rp0 : reps[r];
-  const SweepDev sw = swp[r];
+  const SweepDev sw = ONE ? sw0 : swp[r];
   const int colour = (int)((sw.order >> (4 * phase)) & 0xFu);
   const double shx = sw.shx, shy = sw.shy, shz = sw.shz;
 
-  // active cell of this work item
+  // active cell of this work item.  No exit before the count loads: a
+  // branch there would hold the kernel-argument, sweep-counter and table
+  // loads behind it, one round trip each.  A lane group past the last
+  // active cell or in the padding of its replica's range loads no count
+  // and leaves with them (nc = 0).
   const int hz2 = rp.gz >> 1, hy2 = rp.gy >> 1;
   const int la = a - rp.aoff;
-  if (la >= hz2 * hy2 * (rp.gx >> 1)) return;  // padding of the replica's range
+  const bool live = a < A && la < hz2 * hy2 * (rp.gx >> 1);
   const int iz2 = la % hz2;
   const int t2 = la / hz2;
   const int iy2 = t2 % hy2;
@@ -201,7 +205,7 @@ __device__ __forceinline__ void phase_cell(
 
   // lane j < 27 owns neighbour cell (j/9-1, (j/3)%3-1, j%3-1); 13 is self
   int nj = 0, ncell = 0;
-  if (lane < 27) {
+  if (lane < 27 && live) {
     const int dx = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dz = lane % 3 - 1;
     if (stencil_axis_ok(dx, rp.gx) && stencil_axis_ok(dy, rp.gy) && stencil_axis_ok(dz, rp.gz)) {
       ncell = rp.coff + (wrap_index(cx + dx, rp.gx) * rp.gy + wrap_index(cy + dy, rp.gy)) * rp.gz +
@@ -254,17 +258,18 @@ __device__ __forceinline__ void phase_cell(
   // work item in flight together
   for (int q = 0; q < nj; ++q) Lm[off + q] = q * C + ncell;
   wave_lds_sync();
+  // Entries past T read entry 0's record (T >= nc >= 1 here), so no copy's
+  // loads sit in a branch: a conditional load's value must be moved into
+  // place before the branch joins, which waits for it and holds the next
+  // copy's loads back by a round trip.  Copies past T are never tested.
   dvec<M> px, py, pz;
 #pragma unroll
   for (int j = 0; j < M; ++j) {
     const int q = lane + j * LPC;
-    px[j] = py[j] = pz[j] = 0.0;
-    if (q < T) {
-      const SlotRec *rec = slot + Lm[q];
-      px[j] = rec->x;
-      py[j] = rec->y;
-      pz[j] = rec->z;
-    }
+    const SlotRec *rec = slot + Lm[q < T ? q : 0];
+    px[j] = rec->x;
+    py[j] = rec->y;
+    pz[j] = rec->z;
   }
   if (lane < nc) {
     Sp[lane] = my_id;
@@ -428,13 +433,15 @@ k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__res
         SlotRec *__restrict__ slot, unsigned int *__restrict__ cacc,
         unsigned int *__restrict__ cleft, unsigned int *__restrict__ err) {
   extern __shared__ __attribute__((aligned(16))) int smem_i[];
+  const unsigned long long sweep = *sweep_base + (unsigned long long)k;
   const int cpb = blockDim.x / LPC;
+  SweepDev sw0{};
+  if constexpr (ONE) sw0 = swp[0];
   const int lane = threadIdx.x & (LPC - 1);
   const int sub = threadIdx.x / LPC;
   phase_cell<CAP, LPC, LCAP, ONE>(blockIdx.x * cpb + sub, blockIdx.x * cpb, lane,
-                                  smem_i + (size_t)sub * (LCAP + 14 * CAP), reps, rp0, swp, arep, A,
-                                  phase, *sweep_base + (unsigned long long)k, x, y, z, cnt, C, slot,
-                                  cacc, cleft, err);
+                                  smem_i + (size_t)sub * (LCAP + 14 * CAP), reps, rp0, swp, sw0,
+                                  arep, A, phase, sweep, x, y, z, cnt, C, slot, cacc, cleft, err);
 }
 
 // Barrier of a cooperative launch (every workgroup resident): an arrival
@@ -508,8 +515,9 @@ k_sweeps_persistent(const RepDev *__restrict__ reps, const RepDev rp0, SweepDev 
     for (int p = 0; p < 8; ++p) {
       for (int a0 = blockIdx.x * cpb; a0 < A; a0 += gridDim.x * cpb) {
         wave_lds_sync();  // the group's LDS region is reused cell after cell
-        phase_cell<CAP, LPC, LCAP, ONE>(a0 + sub, a0, lane, Lm, reps, rp0, swp, arep, A, p, sweep, x,
-                                        y, z, cnt, C, slot, cacc, cleft, err);
+        const SweepDev sw0 = ONE ? swp[0] : SweepDev{};
+        phase_cell<CAP, LPC, LCAP, ONE>(a0 + sub, a0, lane, Lm, reps, rp0, swp, sw0, arep, A, p,
+                                        sweep, x, y, z, cnt, C, slot, cacc, cleft, err);
       }
       if (!grid_barrier(bar, err)) return;
     }

@@ -228,6 +228,37 @@ def test_proposal_scores_every_voxel_when_voxels_are_large():
     ctx.close()
 
 
+def test_proposal_with_no_free_voxel():
+    """fcc at rho = 1.2 (nearest neighbours 1.056 apart): every voxel centre
+    lies within 0.65 of a sphere, below the exclusion radius 0.784 of voxels
+    of edge 0.249, so no voxel is free; every point of the box lies within
+    0.75 of a sphere, so no insertion can succeed.  The proposal keeps the
+    uniform part alone: all n insertions are made, none hits, and the
+    estimate is exactly 0 at every mixing weight.  A recomputation after
+    forget() gives the cached call's result."""
+    lat, box = gpu.lattice(2, (4, 4, 4), 1.2)
+    ctx = gpu.Context([dict(n=len(lat), box=box, dr_max=0.05, seed=3)])
+    ctx.upload(lat)
+    net = gpu.Mlp(ctx, seed=2)
+    f, n_free = net.proposal_scores(0.25)
+    assert len(f) == 24 ** 3 and n_free == 0 and np.isneginf(f).all()
+    for mix in (0.0, 0.5, 0.9):
+        r = net.widom(0.25, mix, 7, 1, 100000)
+        assert r["n"] == 100000 and r["hits"] == 0 and r["estimate"] == 0.0
+    assert ctx.widom_uniform(7, 1, 0, 100000) == 0
+    # a free configuration after sweeps at a lower density: the cached
+    # estimate equals a cold recomputation bit for bit
+    ctx2, _ = _ctx_with(6, 0.6, 30)
+    net2 = gpu.Mlp(ctx2, seed=2)
+    first = net2.widom(0.25, 0.9, 7, 1, 100000)
+    cached = net2.widom(0.25, 0.9, 7, 1, 100000)
+    net2.forget()
+    cold = net2.widom(0.25, 0.9, 7, 1, 100000)
+    assert first == cached == cold and first["hits"] > 0
+    ctx2.close()
+    ctx.close()
+
+
 def test_two_networks_share_a_context():
     """Two networks on one context alternate learned Widom calls with
     voxel_scores calls, all of which write the context's shared score and
