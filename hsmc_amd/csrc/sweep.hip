@@ -135,6 +135,23 @@ __device__ __forceinline__ double wave_lane(double v, int t) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// Inclusive sum of v over each group of LPC (32 or 64) consecutive lanes of
+// a full wave, in DPP moves with no LDS round trip: shifts by 1, 2, 4 and 8
+// within each row of 16 lanes, then lane 15 of each even row added to the
+// row above it (the two halves of a 32-lane group) and, for 64-lane groups,
+// lane 31 added to rows 2 and 3.  A shifted-in lane outside the row reads 0.
+template <int LPC>
+__device__ __forceinline__ int group_inclusive_sum(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);   // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);   // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);   // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);   // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+  if constexpr (LPC == 64)
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+  return v;
+}
+
 // A lane's copies of its share of the stencil list (entries lane,
 // lane + LPC, ...): a vector value, so variable indexing stays in registers.
 template <int M>
@@ -196,30 +213,30 @@ __device__ __forceinline__ void phase_cell(
   const int cz = 2 * iz2 + ((colour >> 2) & 1);
   const int cid = (cx * rp.gy + cy) * rp.gz + cz;
 
+  // Loads are issued ahead of any work that does not need their values, and
+  // without a branch around them: the compiler keeps the order of these
+  // lane-masked blocks, and a value loaded inside one is waited for before
+  // the block ends.  Lane j < 27 owns neighbour cell (j/9-1, (j/3)%3-1,
+  // j%3-1), 13 being the active cell itself; the other lanes read the count
+  // of cell 0 and discard it.
+  const int sdx = lane / 9 - 1, sdy = (lane / 3) % 3 - 1, sdz = lane % 3 - 1;
+  const bool owner = lane < 27 && live && stencil_axis_ok(sdx, rp.gx) &&
+                     stencil_axis_ok(sdy, rp.gy) && stencil_axis_ok(sdz, rp.gz);
+  const int ncell =
+      owner ? rp.coff + (wrap_index(cx + sdx, rp.gx) * rp.gy + wrap_index(cy + sdy, rp.gy)) * rp.gz +
+                  wrap_index(cz + sdz, rp.gz)
+            : 0;
+  const int n_loaded = cntb[ncell];
+
   // draw block of attempt t = lane for every attempt a cell can make (t <
-  // CAP), computed while the counts below are in flight
+  // CAP), computed while the counts are in flight
   uint32_t w[4] = {0u, 0u, 0u, 0u};
   if (lane < CAP)
     philox4x32_10((uint32_t)cid, (uint32_t)lane, (uint32_t)sweep, (uint32_t)(sweep >> 32), rp.k0,
                   rp.k1, w);
 
-  // lane j < 27 owns neighbour cell (j/9-1, (j/3)%3-1, j%3-1); 13 is self
-  int nj = 0, ncell = 0;
-  if (lane < 27 && live) {
-    const int dx = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dz = lane % 3 - 1;
-    if (stencil_axis_ok(dx, rp.gx) && stencil_axis_ok(dy, rp.gy) && stencil_axis_ok(dz, rp.gz)) {
-      ncell = rp.coff + (wrap_index(cx + dx, rp.gx) * rp.gy + wrap_index(cy + dy, rp.gy)) * rp.gz +
-              wrap_index(cz + dz, rp.gz);
-      nj = cntb[ncell];
-      if (nj > CAP) nj = CAP;
-    }
-  }
-  int incl = nj;
-#pragma unroll
-  for (int d = 1; d < LPC; d <<= 1) {
-    const int v = __shfl_up(incl, d, LPC);
-    if (lane >= d) incl += v;
-  }
+  const int nj = owner ? (n_loaded < CAP ? n_loaded : CAP) : 0;
+  const int incl = group_inclusive_sum<LPC>(nj);
   const int off = incl - nj;
   const int T = __shfl(incl, LPC - 1, LPC);
   const int nc = __shfl(nj, 13, LPC);
@@ -230,38 +247,18 @@ __device__ __forceinline__ void phase_cell(
     return;
   }
 
-  // lane t < nc draws attempt t: occupant rank (rng_get_int rule) and the
-  // displacement (r - 0.5) * dr_max per axis, off the serial path below
-  uint32_t my_k = 0;
-  double my_dx = 0.0, my_dy = 0.0, my_dz = 0.0;
-  if (lane < nc) {
-    my_k = occupant_pick(w[0], (uint32_t)nc, (uint32_t)cid, (uint32_t)lane, sweep, rp.k0, rp.k1);
-    my_dx = (u32_unit_closed(w[1]) - 0.5) * rp.dr_max;
-    my_dy = (u32_unit_closed(w[2]) - 0.5) * rp.dr_max;
-    my_dz = (u32_unit_closed(w[3]) - 0.5) * rp.dr_max;
-  }
-
-  // the active cell's occupants first (slot t of the active cell for lane
-  // t < nc: no slot map needed), so their ranking and the attempt
-  // preparation below overlap the neighbour loads
-  int my_id = 0;
-  double o_x = 0.0, o_y = 0.0, o_z = 0.0;
-  if (lane < nc) {
-    const SlotRec *rec = slot + (size_t)lane * C + rp.coff + cid;
-    my_id = rec->id;
-    o_x = rec->x;
-    o_y = rec->y;
-    o_z = rec->z;
-  }
+  // the active cell's occupants (slot t of the active cell for lane t < nc,
+  // slot 0 for the other lanes: no slot map needed), so their ranking and the
+  // attempt preparation below overlap the neighbour loads
+  const SlotRec *orec = slot + (size_t)(lane < nc ? lane : 0) * C + rp.coff + cid;
+  const int my_id = orec->id;
+  const double o_x = orec->x, o_y = orec->y, o_z = orec->z;
   // stage the stencil list: the slot of every entry into LDS, then each
   // lane loads its entries' FP64 positions into registers, all loads of the
-  // work item in flight together
+  // work item in flight together.  Entries past T read entry 0's record (T
+  // >= nc >= 1 here); copies past T are never tested.
   for (int q = 0; q < nj; ++q) Lm[off + q] = q * C + ncell;
   wave_lds_sync();
-  // Entries past T read entry 0's record (T >= nc >= 1 here), so no copy's
-  // loads sit in a branch: a conditional load's value must be moved into
-  // place before the branch joins, which waits for it and holds the next
-  // copy's loads back by a round trip.  Copies past T are never tested.
   dvec<M> px, py, pz;
 #pragma unroll
   for (int j = 0; j < M; ++j) {
@@ -270,6 +267,17 @@ __device__ __forceinline__ void phase_cell(
     px[j] = rec->x;
     py[j] = rec->y;
     pz[j] = rec->z;
+  }
+
+  // while the records are in flight, lane t < nc draws attempt t: occupant
+  // rank (rng_get_int rule) and the displacement (r - 0.5) * dr_max per axis
+  uint32_t my_k = 0;
+  double my_dx = 0.0, my_dy = 0.0, my_dz = 0.0;
+  if (lane < nc) {
+    my_k = occupant_pick(w[0], (uint32_t)nc, (uint32_t)cid, (uint32_t)lane, sweep, rp.k0, rp.k1);
+    my_dx = (u32_unit_closed(w[1]) - 0.5) * rp.dr_max;
+    my_dy = (u32_unit_closed(w[2]) - 0.5) * rp.dr_max;
+    my_dz = (u32_unit_closed(w[3]) - 0.5) * rp.dr_max;
   }
   if (lane < nc) {
     Sp[lane] = my_id;
