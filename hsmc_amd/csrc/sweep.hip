@@ -71,8 +71,13 @@ __device__ __forceinline__ uint32_t sweep_order(const RepDev &rp, unsigned long 
 }
 
 // Work item i of a sweep's rebin: clear cell i of the next sweep's count
-// buffer, draw replica i's shift and sublattice order, and file particle i
-// into its cell of the shifted grid.
+// buffer, file particle i into its cell of the shifted grid, and draw
+// replica i's shift and sublattice order.  The particle's position (and, for
+// several replicas, its replica index) is loaded first, so the shift's draw
+// runs while the loads are in flight; one replica's record is the kernel
+// argument (no per-lane loads of it); replica i's table entry is drawn last,
+// while the slot index's atomic is in flight.
+template <bool ONE>
 __device__ __forceinline__ void rebin_one(int i, const RepDev *__restrict__ reps, const RepDev &rp0,
                                           SweepDev *__restrict__ swp, int R,
                                           const int *__restrict__ prep, int P, int C,
@@ -81,38 +86,42 @@ __device__ __forceinline__ void rebin_one(int i, const RepDev *__restrict__ reps
                                           SlotRec *__restrict__ slot, int cap,
                                           unsigned long long sweep, unsigned *__restrict__ err) {
   const int buf = (int)(sweep & 1ull);
+  const int ip = i < P ? i : 0;  // a work item past the particles reads particle 0 and drops it
+  const double px = x[ip], py = y[ip], pz = z[ip];
+  const int rr = ONE ? 0 : prep[ip];
   if (i < C) cnt[(size_t)(buf ^ 1) * C + i] = 0;  // next sweep's buffer
+  if (i < P) {
+    const RepDev rp = ONE ? rp0 : reps[rr];
+    double shx, shy, shz;
+    sweep_shift(rp, sweep, shx, shy, shz);
+    const int cx = shifted_cell(px, shx, rp.lx, rp.ix, rp.gx);
+    const int cy = shifted_cell(py, shy, rp.ly, rp.iy, rp.gy);
+    const int cz = shifted_cell(pz, shz, rp.lz, rp.iz, rp.gz);
+    const int c = rp.coff + (cx * rp.gy + cy) * rp.gz + cz;
+    const int sl = atomicAdd(&cnt[(size_t)buf * C + c], 1);
+    if (sl < cap) {
+      SlotRec rec;
+      rec.x = px;
+      rec.y = py;
+      rec.z = pz;
+      rec.id = i - rp.poff;
+      rec.pad = 0;
+      slot[(size_t)sl * C + c] = rec;  // slot-major: slot q of every cell is one array
+    } else {
+      atomicOr(err, kErrOverflow);
+    }
+  }
   if (i < R) {  // replica i's shift and sublattice order, read by this sweep's phases
-    const RepDev ri = reps[i];
+    const RepDev ri = ONE ? rp0 : reps[i];
     SweepDev d;
     sweep_shift(ri, sweep, d.shx, d.shy, d.shz);
     d.order = sweep_order(ri, sweep);
     d.pad = 0u;
     swp[i] = d;
   }
-  if (i >= P) return;
-  const RepDev rp = R == 1 ? rp0 : reps[prep[i]];
-  double shx, shy, shz;
-  sweep_shift(rp, sweep, shx, shy, shz);
-  const double px = x[i], py = y[i], pz = z[i];
-  const int cx = shifted_cell(px, shx, rp.lx, rp.ix, rp.gx);
-  const int cy = shifted_cell(py, shy, rp.ly, rp.iy, rp.gy);
-  const int cz = shifted_cell(pz, shz, rp.lz, rp.iz, rp.gz);
-  const int c = rp.coff + (cx * rp.gy + cy) * rp.gz + cz;
-  const int sl = atomicAdd(&cnt[(size_t)buf * C + c], 1);
-  if (sl < cap) {
-    SlotRec rec;
-    rec.x = px;
-    rec.y = py;
-    rec.z = pz;
-    rec.id = i - rp.poff;
-    rec.pad = 0;
-    slot[(size_t)sl * C + c] = rec;  // slot-major: slot q of every cell is one array
-  } else {
-    atomicOr(err, kErrOverflow);
-  }
 }
 
+template <bool ONE>
 __global__ void __launch_bounds__(256)
 k_rebin(const RepDev *__restrict__ reps, const RepDev rp0, SweepDev *__restrict__ swp, int R,
         const int *__restrict__ prep, int P, int C,
@@ -121,7 +130,7 @@ k_rebin(const RepDev *__restrict__ reps, const RepDev rp0, SweepDev *__restrict_
         const unsigned long long *__restrict__ sweep_base, int k, unsigned *__restrict__ err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const unsigned long long sweep = *sweep_base + (unsigned long long)(long long)k;
-  rebin_one(i, reps, rp0, swp, R, prep, P, C, x, y, z, cnt, slot, cap, sweep, err);
+  rebin_one<ONE>(i, reps, rp0, swp, R, prep, P, C, x, y, z, cnt, slot, cap, sweep, err);
 }
 
 // Value of v in lane t of the wave, for a t that is the same in every lane:
@@ -518,7 +527,7 @@ k_sweeps_persistent(const RepDev *__restrict__ reps, const RepDev rp0, SweepDev 
   for (int s = 0; s < n_sweeps; ++s) {
     const unsigned long long sweep = base + (unsigned long long)s;
     for (int i = gtid; i < work; i += gstride)
-      rebin_one(i, reps, rp0, swp, R, prep, P, C, x, y, z, cnt, slot, cap, sweep, err);
+      rebin_one<ONE>(i, reps, rp0, swp, R, prep, P, C, x, y, z, cnt, slot, cap, sweep, err);
     if (!grid_barrier(bar, err)) return;
     for (int p = 0; p < 8; ++p) {
       for (int a0 = blockIdx.x * cpb; a0 < A; a0 += gridDim.x * cpb) {
@@ -659,7 +668,8 @@ bool tuning_ok(const hsmc_gpu_ctx *ctx, const hsmc_gpu_tuning &t) {
 // Enqueue the kernels of sweep k of the current batch on stream s.
 void enqueue_sweep(hsmc_gpu_ctx *ctx, int k, hipStream_t s, hipEvent_t *phase_events) {
   const long work = std::max(ctx->Ptot, ctx->Ctot);
-  hipLaunchKernelGGL(k_rebin, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, ctx->d_reps,
+  hipLaunchKernelGGL(ctx->R == 1 ? k_rebin<true> : k_rebin<false>,
+                     dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, ctx->d_reps,
                      ctx->rep0, ctx->d_swp, ctx->R, ctx->d_prep, (int)ctx->Ptot, (int)ctx->Ctot, ctx->d_x, ctx->d_y, ctx->d_z,
                      ctx->d_cnt, ctx->d_slot, ctx->cap,
                      ctx->d_sweep, k, ctx->d_err);
@@ -782,7 +792,8 @@ int rebin_last_sweep(hsmc_gpu_ctx *ctx) {
   HSMC_HIP(ctx, hipMemsetAsync(ctx->d_cnt + (size_t)buf * ctx->Ctot, 0, sizeof(int) * ctx->Ctot,
                                ctx->stream));
   const long work = std::max(ctx->Ptot, ctx->Ctot);
-  hipLaunchKernelGGL(k_rebin, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, ctx->stream,
+  hipLaunchKernelGGL(ctx->R == 1 ? k_rebin<true> : k_rebin<false>,
+                     dim3((unsigned)((work + 255) / 256)), dim3(256), 0, ctx->stream,
                      ctx->d_reps, ctx->rep0, ctx->d_swp, ctx->R, ctx->d_prep, (int)ctx->Ptot, (int)ctx->Ctot, ctx->d_x, ctx->d_y,
                      ctx->d_z, ctx->d_cnt, ctx->d_slot, ctx->cap,
                      ctx->d_sweep, -1, ctx->d_err);
