@@ -312,38 +312,91 @@ struct NGrid {  // CSR observable grid (side >= 1.5) over one replica
   const double *px, *py, *pz;
 };
 
+// Candidate range of one row (dx, dy) of the 27-cell stencil: the row's z
+// cells hold consecutive entries of the CSR arrays, so the row is one range
+// [a0, a1), plus a second [b0, b1) where the stencil wraps around z (the
+// cell at the far end).  With fewer than 3 cells along z the row is every
+// cell, once.
+__device__ __forceinline__ void row_ranges(const NGrid &g, int base, int cz, int &a0, int &a1,
+                                           int &b0, int &b1) {
+  int ac0, ac1, bc0 = 0, bc1 = 0;  // cell index ranges [ac0, ac1), [bc0, bc1) within the row
+  if (g.nz >= 3) {
+    if (cz == 0) {
+      ac0 = 0, ac1 = 2, bc0 = g.nz - 1, bc1 = g.nz;
+    } else if (cz == g.nz - 1) {
+      ac0 = g.nz - 2, ac1 = g.nz, bc0 = 0, bc1 = 1;
+    } else {
+      ac0 = cz - 1, ac1 = cz + 2;
+    }
+  } else {
+    ac0 = 0, ac1 = g.nz;
+  }
+  // four loads with no branch around them (an empty second range reads one
+  // entry twice), so the rows' loads all issue before any is waited for
+  a0 = g.start[base + ac0];
+  a1 = g.start[base + ac1];
+  b0 = g.start[base + bc0];
+  b1 = g.start[base + bc1];
+}
+
 // 12 smallest squared distances (ascending) from q to sphere centres in the
-// 27 cells around q; entries not found stay at `far2`.
+// 27 cells around q; entries not found stay at `far2`.  The result depends
+// only on the set of candidates, not on the order they are met, so the
+// search takes them row by row: the rows' ranges are loaded together, then
+// each range in chunks of 8 spheres whose positions are loaded together
+// (a chunk's loads are unconditional, past the range's end they repeat its
+// first sphere), so a lane waits on about 20 load round trips rather than
+// on one per cell and one per sphere.
+__device__ __forceinline__ void take12(float d2, float best[NIN]) {
+  if (d2 < best[NIN - 1]) {
+    best[NIN - 1] = d2;
+#pragma unroll
+    for (int k = NIN - 1; k > 0; --k) {
+      const float lo = fminf(best[k - 1], best[k]), hi = fmaxf(best[k - 1], best[k]);
+      best[k - 1] = lo;
+      best[k] = hi;
+    }
+  }
+}
+
 __device__ __forceinline__ void nearest12(const NGrid &g, double qx, double qy, double qz,
                                           float far2, float best[NIN]) {
+  constexpr int CH = 8;
 #pragma unroll
   for (int k = 0; k < NIN; ++k) best[k] = far2;
   const int cx = plain_cell(qx, g.sx, g.nx), cy = plain_cell(qy, g.sy, g.ny),
             cz = plain_cell(qz, g.sz, g.nz);
-  for (int dx = -1; dx <= 1; ++dx) {
-    if (!stencil_axis_ok(dx, g.nx)) continue;
-    const int ax = wrap_index(cx + dx, g.nx);
-    for (int dy = -1; dy <= 1; ++dy) {
-      if (!stencil_axis_ok(dy, g.ny)) continue;
-      const int ay = wrap_index(cy + dy, g.ny);
-      for (int dz = -1; dz <= 1; ++dz) {
-        if (!stencil_axis_ok(dz, g.nz)) continue;
-        const int c = (ax * g.ny + ay) * g.nz + wrap_index(cz + dz, g.nz);
-        for (int p = g.start[c]; p < g.start[c + 1]; ++p) {
-          const float d2 = (float)dist2(qx, qy, qz, g.px[p], g.py[p], g.pz[p], g.lx, g.ly, g.lz,
-                                        g.hx, g.hy, g.hz);
-          if (d2 < best[NIN - 1]) {
-            best[NIN - 1] = d2;
+  int ra0[9], ra1[9], rb0[9], rb1[9];
 #pragma unroll
-            for (int k = NIN - 1; k > 0; --k) {
-              const float lo = fminf(best[k - 1], best[k]), hi = fmaxf(best[k - 1], best[k]);
-              best[k - 1] = lo;
-              best[k] = hi;
-            }
-          }
-        }
+  for (int r = 0; r < 9; ++r) {
+    const int dx = r / 3 - 1, dy = r % 3 - 1;
+    const bool ok = stencil_axis_ok(dx, g.nx) && stencil_axis_ok(dy, g.ny);
+    const int base = ok ? (wrap_index(cx + dx, g.nx) * g.ny + wrap_index(cy + dy, g.ny)) * g.nz : 0;
+    row_ranges(g, base, cz, ra0[r], ra1[r], rb0[r], rb1[r]);
+    if (!ok) ra1[r] = ra0[r], rb1[r] = rb0[r];  // a row the stencil leaves out
+  }
+  auto scan = [&](int p0, int p1) {
+    for (int p = p0; p < p1; p += CH) {
+      double ox[CH], oy[CH], oz[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int q = p + u < p1 ? p + u : p0;
+        ox[u] = g.px[q];
+        oy[u] = g.py[q];
+        oz[u] = g.pz[q];
+      }
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        if (p + u < p1)
+          take12((float)dist2(qx, qy, qz, ox[u], oy[u], oz[u], g.lx, g.ly, g.lz, g.hx, g.hy, g.hz),
+                 best);
       }
     }
+  };
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    scan(ra0[r], ra1[r]);
+    scan(rb0[r], rb1[r]);
   }
 }
 
