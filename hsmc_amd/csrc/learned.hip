@@ -888,24 +888,48 @@ __global__ void k_cdf_add(double *__restrict__ cdf, long V, const double *__rest
   if (i < V) cdf[i] += boff[i / 4096];
 }
 
+// No sphere centre closer than 1 to q in the 27 cells around it (the
+// overlap grid's cells are at least 1 wide).  The answer does not depend
+// on the order of the cells, so they are visited nearest first (own cell,
+// then the 6 sharing a face, with their ranges loaded together, then the
+// rest): almost every proposed point overlaps a sphere, and usually one of
+// those.
 __device__ __forceinline__ bool widom_free(const NGrid &g1, double qx, double qy, double qz) {
   const int cx = plain_cell(qx, g1.sx, g1.nx), cy = plain_cell(qy, g1.sy, g1.ny),
             cz = plain_cell(qz, g1.sz, g1.nz);
-  for (int dx = -1; dx <= 1; ++dx) {
-    if (!stencil_axis_ok(dx, g1.nx)) continue;
-    const int ax = wrap_index(cx + dx, g1.nx);
-    for (int dy = -1; dy <= 1; ++dy) {
-      if (!stencil_axis_ok(dy, g1.ny)) continue;
-      const int ay = wrap_index(cy + dy, g1.ny);
-      for (int dz = -1; dz <= 1; ++dz) {
-        if (!stencil_axis_ok(dz, g1.nz)) continue;
-        const int c = (ax * g1.ny + ay) * g1.nz + wrap_index(cz + dz, g1.nz);
-        for (int p = g1.start[c]; p < g1.start[c + 1]; ++p)
-          if (dist2(qx, qy, qz, g1.px[p], g1.py[p], g1.pz[p], g1.lx, g1.ly, g1.lz, g1.hx, g1.hy,
-                    g1.hz) < 1.0)
-            return false;
-      }
-    }
+  auto cell_of = [&](int j, bool &ok) {
+    int dx, dy, dz;
+    stencil_by_distance(j, dx, dy, dz);
+    ok = stencil_axis_ok(dx, g1.nx) && stencil_axis_ok(dy, g1.ny) && stencil_axis_ok(dz, g1.nz);
+    return ok ? (wrap_index(cx + dx, g1.nx) * g1.ny + wrap_index(cy + dy, g1.ny)) * g1.nz +
+                    wrap_index(cz + dz, g1.nz)
+              : 0;
+  };
+  auto hit = [&](int b, int e) {
+    for (int p = b; p < e; ++p)
+      if (dist2(qx, qy, qz, g1.px[p], g1.py[p], g1.pz[p], g1.lx, g1.ly, g1.lz, g1.hx, g1.hy,
+                g1.hz) < 1.0)
+        return true;
+    return false;
+  };
+  int c7[7], b7[7], e7[7];
+  bool ok7[7];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) c7[j] = cell_of(j, ok7[j]);
+#pragma unroll
+  for (int j = 0; j < 7; ++j) b7[j] = g1.start[c7[j]];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) e7[j] = g1.start[c7[j] + 1];
+#pragma unroll
+  for (int j = 0; j < 7; ++j)
+    if (!ok7[j]) e7[j] = b7[j];
+#pragma unroll
+  for (int j = 0; j < 7; ++j)
+    if (hit(b7[j], e7[j])) return false;
+  for (int j = 7; j < 27; ++j) {
+    bool ok;
+    const int c = cell_of(j, ok);
+    if (ok && hit(g1.start[c], g1.start[c + 1])) return false;
   }
   return true;
 }

@@ -190,18 +190,55 @@ __device__ __forceinline__ bool for_stencil(const Grid &g, int cx, int cy, int c
   return false;
 }
 
+// Any sphere other than skip_id closer than 1 to q, in the 27 cells around
+// q (check_overlap, moves.c:157-212, and widom_check_overlap,
+// compute_widom_chem_pot.c:82-120).  The answer does not depend on the
+// order of the cells, so they are visited nearest first (the point's own
+// cell, then the 6 sharing a face, then the rest), with the ranges of the
+// first 7 loaded together: a point that overlaps usually does so there, and
+// leaves after one round trip for the ranges and one for the spheres.
 __device__ __forceinline__ bool point_overlaps(const Grid &g, double qx, double qy, double qz,
                                                int skip_id) {
   const int cx = plain_cell(qx, g.sx, g.nx), cy = plain_cell(qy, g.sy, g.ny),
             cz = plain_cell(qz, g.sz, g.nz);
-  return for_stencil(g, cx, cy, cz, [&](int b, int e) {
+  auto cell_of = [&](int j, bool &ok) {
+    int dx, dy, dz;
+    stencil_by_distance(j, dx, dy, dz);
+    ok = stencil_axis_ok(dx, g.nx) && stencil_axis_ok(dy, g.ny) && stencil_axis_ok(dz, g.nz);
+    return ok ? (wrap_index(cx + dx, g.nx) * g.ny + wrap_index(cy + dy, g.ny)) * g.nz +
+                    wrap_index(cz + dz, g.nz)
+              : 0;
+  };
+  auto scan = [&](int b, int e) {
     for (int p = b; p < e; ++p) {
-      if (g.ids[p] == skip_id) continue;
-      if (dist2(qx, qy, qz, g.px[p], g.py[p], g.pz[p], g.lx, g.ly, g.lz, g.hx, g.hy, g.hz) < 1.0)
-        return true;
+      // the id and the position load together; the skip is tested after
+      const int id = g.ids[p];
+      const double d2 =
+          dist2(qx, qy, qz, g.px[p], g.py[p], g.pz[p], g.lx, g.ly, g.lz, g.hx, g.hy, g.hz);
+      if (id != skip_id && d2 < 1.0) return true;
     }
     return false;
-  });
+  };
+  int c7[7], b7[7], e7[7];
+  bool ok7[7];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) c7[j] = cell_of(j, ok7[j]);
+#pragma unroll
+  for (int j = 0; j < 7; ++j) b7[j] = g.start[c7[j]];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) e7[j] = g.start[c7[j] + 1];
+#pragma unroll
+  for (int j = 0; j < 7; ++j)
+    if (!ok7[j]) e7[j] = b7[j];  // a cell the stencil leaves out (tiny grids): empty
+#pragma unroll
+  for (int j = 0; j < 7; ++j)
+    if (scan(b7[j], e7[j])) return true;
+  for (int j = 7; j < 27; ++j) {
+    bool ok;
+    const int c = cell_of(j, ok);
+    if (ok && scan(g.start[c], g.start[c + 1])) return true;
+  }
+  return false;
 }
 
 __global__ void k_overlap_batch(Grid g, const int *__restrict__ idx, const double *__restrict__ q,
