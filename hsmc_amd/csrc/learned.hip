@@ -628,10 +628,10 @@ k_voxel_scores_tiled(NGrid g, Vox vx, float far2, double smin, const MlpDev *__r
 // every voxel whose centre's squared distance to it, in FP64 as
 // compute_dist evaluates it (minimum image), is below r^2 -- so a voxel is
 // marked exactly when some sphere blocks it; all marks store the same 1.
-// k_free_tiles: one workgroup per tile of 4 x 8 x 8 voxels puts the unmarked
-// ("free") voxels in the tile's own range of `slots` (256 per tile, in
-// thread order) with their number in cnt[tile]; k_gather_free packs them
-// into one list (tile order, then thread order) for k_voxel_scores_list.
+// k_free_tiles: one wave per tile of 4 x 8 x 8 voxels puts the unmarked
+// ("free") voxels in the tile's own range of `slots` (256 per tile, in the
+// tile's x, y, z order) with their number in cnt[tile]; k_gather_free packs
+// them into one list (tile order, then that order) for k_voxel_scores_list.
 // No global atomics: the tile counts are prefix-summed instead.  Scores are
 // written for free voxels only; the flags tell the others apart.
 struct Box3 {
@@ -652,6 +652,14 @@ __device__ __forceinline__ int wrap_voxel(int i, int m) {
   return i < 0 ? i + m : (i >= m ? i - m : i);
 }
 
+// Lane t's value of v, read into every lane (t the same in all of them).
+__device__ __forceinline__ double lane_value(double v, int t) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, t);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), t);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 __global__ void __launch_bounds__(256)
 k_mark_blocked(const double *__restrict__ x, const double *__restrict__ y,
                const double *__restrict__ z, int N, Vox vx, Box3 b, double r,
@@ -663,48 +671,64 @@ k_mark_blocked(const double *__restrict__ x, const double *__restrict__ y,
   const int lx = voxel_first(px, r, vx.vx), ly = voxel_first(py, r, vx.vy),
             lz = voxel_first(pz, r, vx.vz);
   const double r2 = r * r;
-  // 8 x 8 x 8 candidates: lane's (y, z) column, 8 x-layers
+  // 8 x 8 x 8 candidates: lane's (y, z) column, 8 x-layers.  The squared
+  // distance is compute_dist's (dx^2 + dy^2) + dz^2 with the minimum image
+  // per axis.  Its x part is the same in every lane, so lane k < 8 computes
+  // layer k's index and dx^2 once and each layer reads them from that lane.
+  const int kx = lane & 7;
+  const int ixk = wrap_voxel(lx + kx, vx.mx);
+  const double dxk = min_image((ixk + 0.5) * vx.vx - px, b.lx, b.hx);
+  const double dx2k = dxk * dxk;
   const int iy = wrap_voxel(ly + (lane >> 3), vx.my), iz = wrap_voxel(lz + (lane & 7), vx.mz);
-  const double cy = (iy + 0.5) * vx.vy, cz = (iz + 0.5) * vx.vz;
-#pragma unroll 2
+  const double dy = min_image((iy + 0.5) * vx.vy - py, b.ly, b.hy);
+  const double dz = min_image((iz + 0.5) * vx.vz - pz, b.lz, b.hz);
+  const double dy2 = dy * dy, dz2 = dz * dz;
+  const long col = (long)iy * vx.mz + iz, layer = (long)vx.my * vx.mz;
+#pragma unroll
   for (int k = 0; k < 8; ++k) {
-    const int ix = wrap_voxel(lx + k, vx.mx);
-    const double cx = (ix + 0.5) * vx.vx;
-    if (dist2(cx, cy, cz, px, py, pz, b.lx, b.ly, b.lz, b.hx, b.hy, b.hz) < r2)
-      flag[((long)ix * vx.my + iy) * vx.mz + iz] = 1;
+    const double dx2 = lane_value(dx2k, k);
+    const int ix = __builtin_amdgcn_readlane(ixk, k);
+    if (dx2 + dy2 + dz2 < r2) flag[(long)ix * layer + col] = 1;
   }
 }
 
 __global__ void __launch_bounds__(256)
 k_free_tiles(Vox vx, const unsigned char *__restrict__ flag, int *__restrict__ slots,
-             int *__restrict__ cnt) {
-  __shared__ int wcount[4];
+             int *__restrict__ cnt, long tiles) {
+  const long tile = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (tile >= tiles) return;  // the whole wave
   const int ntz = (vx.mz + TZ - 1) / TZ, nty = (vx.my + TY - 1) / TY;
-  const int bz = blockIdx.x % ntz, by = (blockIdx.x / ntz) % nty, bx = blockIdx.x / (ntz * nty);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int ix = bx * TX + w, iy = by * TY + (lane >> 3), iz = bz * TZ + (lane & 7);
-  const bool mine = ix < vx.mx && iy < vx.my && iz < vx.mz;
-  const long v = ((long)ix * vx.my + iy) * vx.mz + iz;
-  const bool freev = mine && flag[v] == 0;
-  // the tile's free voxels in thread order: a ballot per wave, then the
-  // waves' counts in order
-  const unsigned long long bal = __ballot(freev);
-  if (lane == 0) wcount[w] = __popcll(bal);
-  __syncthreads();
-  int before = 0;
-  for (int k = 0; k < w; ++k) before += wcount[k];
-  if (freev)
-    slots[(size_t)blockIdx.x * 256 + before + __popcll(bal & ((1ull << lane) - 1ull))] = (int)v;
-  if (threadIdx.x == 0) cnt[blockIdx.x] = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+  const int bz = (int)(tile % ntz), by = (int)((tile / ntz) % nty), bx = (int)(tile / ((long)ntz * nty));
+  // lane l: row (x = l / 16, y = (l / 2) % 8) of the tile, voxels z = 4 (l % 2) .. + 3,
+  // so lane order then z is the tile's x, y, z order
+  const int lane = threadIdx.x & 63;
+  const int ix = bx * TX + (lane >> 4), iy = by * TY + ((lane >> 1) & 7), iz0 = bz * TZ + 4 * (lane & 1);
+  const bool row = ix < vx.mx && iy < vx.my;
+  const long v0 = ((long)ix * vx.my + iy) * vx.mz + iz0;
+  unsigned char f[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) f[k] = flag[row && iz0 + k < vx.mz ? v0 + k : 0];  // no branch around the loads
+  int bits = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) bits |= (row && iz0 + k < vx.mz && f[k] == 0) ? 1 << k : 0;
+  const int n = __popc(bits);
+  const int incl = group_inclusive_sum<64>(n);
+  int *out = slots + (size_t)tile * 256 + (incl - n);
+#pragma unroll
+  for (int k = 0, j = 0; k < 4; ++k)
+    if ((bits >> k) & 1) out[j++] = (int)(v0 + k);
+  if (lane == 63) cnt[tile] = incl;
 }
 
 // Pack the tiles' free voxels into one list: tile t's cnt[t] entries go to
-// list[off[t] ...] (off: exclusive prefix sum of cnt).
+// list[off[t] ...] (off: exclusive prefix sum of cnt); one wave per tile.
 __global__ void __launch_bounds__(256)
 k_gather_free(const int *__restrict__ slots, const int *__restrict__ cnt,
-              const int *__restrict__ off, int *__restrict__ list) {
-  const int t = blockIdx.x, k = threadIdx.x;
-  if (k < cnt[t]) list[off[t] + k] = slots[(size_t)t * 256 + k];
+              const int *__restrict__ off, int *__restrict__ list, long tiles) {
+  const long t = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= tiles) return;
+  const int n = cnt[t], o = off[t];
+  for (int k = threadIdx.x & 63; k < n; k += 64) list[o + k] = slots[(size_t)t * 256 + k];
 }
 
 // Network scores of the free voxels list[0..n) (descriptor + network, as
@@ -1251,11 +1275,11 @@ int launch_proposal_scores(hsmc_gpu_ctx *ctx, int replica, const NGrid &g, const
   Box3 b3{rh.p.lx, rh.p.ly, rh.p.lz, rh.p.lx / 2.0, rh.p.ly / 2.0, rh.p.lz / 2.0};
   hipLaunchKernelGGL(k_mark_blocked, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, ctx->stream,
                      ctx->d_x + rh.poff, ctx->d_y + rh.poff, ctx->d_z + rh.poff, N, vx, b3, r, flag);
-  hipLaunchKernelGGL(k_free_tiles, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, vx, flag,
-                     slots, cnt);
+  hipLaunchKernelGGL(k_free_tiles, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, ctx->stream, vx,
+                     flag, slots, cnt, tiles);
   exclusive_scan(ctx->stream, cnt, (int)tiles, off, part);
-  hipLaunchKernelGGL(k_gather_free, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, slots, cnt,
-                     off, list);
+  hipLaunchKernelGGL(k_gather_free, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, ctx->stream,
+                     slots, cnt, off, list, tiles);
   HSMC_HIP(ctx, hipGetLastError());
   int nf = 0;
   HSMC_HIP(ctx, hipMemcpyAsync(&nf, off + tiles, sizeof(nf), hipMemcpyDeviceToHost, ctx->stream));

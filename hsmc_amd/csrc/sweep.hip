@@ -144,23 +144,6 @@ __device__ __forceinline__ double wave_lane(double v, int t) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Inclusive sum of v over each group of LPC (32 or 64) consecutive lanes of
-// a full wave, in DPP moves with no LDS round trip: shifts by 1, 2, 4 and 8
-// within each row of 16 lanes, then lane 15 of each even row added to the
-// row above it (the two halves of a 32-lane group) and, for 64-lane groups,
-// lane 31 added to rows 2 and 3.  A shifted-in lane outside the row reads 0.
-template <int LPC>
-__device__ __forceinline__ int group_inclusive_sum(int v) {
-  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);   // row_shr:1
-  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);   // row_shr:2
-  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);   // row_shr:4
-  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);   // row_shr:8
-  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
-  if constexpr (LPC == 64)
-    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
-  return v;
-}
-
 // A lane's copies of its share of the stencil list (entries lane,
 // lane + LPC, ...): a vector value, so variable indexing stays in registers.
 template <int M>
