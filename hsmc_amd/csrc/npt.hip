@@ -72,25 +72,40 @@ k_vol_overlap(const RepDev *__restrict__ reps, const SweepDev *__restrict__ swp,
   const int cy = shifted_cell(ay, shy, rp.ly, rp.iy, rp.gy);
   const int cz = shifted_cell(az, shz, rp.lz, rp.iz, rp.gz);
   const int me = i - rp.poff;
-  bool ov = false;
-  // d = 13 is the own cell; d = 14..26 are the forward offsets
-  for (int d = 13; d < 27 && !ov; ++d) {
+  // d = 13 is the own cell; d = 14..26 are the forward offsets.  The
+  // answer does not depend on the order of the tests, so the 14 cells'
+  // counts load together, and each cell's first two slots load with the
+  // poll of the replica's flag (an overlap found elsewhere ends the scan).
+  int cell[14], n[14];
+#pragma unroll
+  for (int j = 0; j < 14; ++j) {
+    const int d = 13 + j;
     const int dx = d / 9 - 1, dy = (d / 3) % 3 - 1, dz = d % 3 - 1;
-    if (!stencil_axis_ok(dx, rp.gx) || !stencil_axis_ok(dy, rp.gy) || !stencil_axis_ok(dz, rp.gz))
-      continue;
-    if (d > 13 && __hip_atomic_load(&flags[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-    const int c = rp.coff + (wrap_index(cx + dx, rp.gx) * rp.gy + wrap_index(cy + dy, rp.gy)) * rp.gz +
-                  wrap_index(cz + dz, rp.gz);
-    const int n = min(cntb[c], cap);
-    for (int q = 0; q < n; ++q) {
-      const SlotRec &rec = slot[(size_t)q * C + c];
-      if (d == 13 && rec.id <= me) continue;  // own cell: each pair once, self excluded
-      // compute_dist with sf (moves.c:400-431), reference particle first
-      if (dist2_scaled(ax, ay, az, rec.x, rec.y, rec.z, rp.lx, rp.ly, rp.lz, sf) < 1.0) {
-        ov = true;
-        break;
-      }
-    }
+    const bool ok = stencil_axis_ok(dx, rp.gx) && stencil_axis_ok(dy, rp.gy) && stencil_axis_ok(dz, rp.gz);
+    cell[j] = rp.coff + (ok ? (wrap_index(cx + dx, rp.gx) * rp.gy + wrap_index(cy + dy, rp.gy)) * rp.gz +
+                                  wrap_index(cz + dz, rp.gz)
+                            : 0);
+    n[j] = ok ? 1 : 0;  // a cell the stencil leaves out (tiny grids) is skipped
+  }
+  int m[14];
+#pragma unroll
+  for (int j = 0; j < 14; ++j) m[j] = cntb[cell[j]];  // no branch around the loads
+#pragma unroll
+  for (int j = 0; j < 14; ++j) n[j] = n[j] ? min(m[j], cap) : 0;
+  auto hits = [&](const SlotRec &rec, bool own) {
+    // compute_dist with sf (moves.c:400-431), reference particle first;
+    // in the own cell each pair once, self excluded
+    return !(own && rec.id <= me) &&
+           dist2_scaled(ax, ay, az, rec.x, rec.y, rec.z, rp.lx, rp.ly, rp.lz, sf) < 1.0;
+  };
+  bool ov = false;
+  for (int j = 0; j < 14 && !ov; ++j) {
+    const int c = cell[j];
+    const SlotRec r0 = slot[c], r1 = slot[(size_t)(cap > 1 ? 1 : 0) * C + c];
+    const int stop = j > 0 ? __hip_atomic_load(&flags[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+    if (stop) break;
+    ov = (n[j] > 0 && hits(r0, j == 0)) || (n[j] > 1 && hits(r1, j == 0));
+    for (int q = 2; q < n[j] && !ov; ++q) ov = hits(slot[(size_t)q * C + c], j == 0);
   }
   if (ov) atomicOr(&flags[r], 1);
 }
