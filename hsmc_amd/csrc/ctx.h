@@ -133,7 +133,9 @@ struct hsmc_gpu_ctx {
   // scratch buffers: every call that writes them takes a new value, and a
   // network's cached scores are valid only while the stamp is the one it took
   unsigned long long score_stamp = 0;
-  // neighbour grids of the learned proposal (descriptor grid, overlap grid)
+  // neighbour-grid cache of the learned proposal: entry 0 the descriptor
+  // grid alone, entry 1 the descriptor and overlap grids built together (its
+  // cut is the sum of the two cuts)
   int ngrid_replica[2] = {-1, -1};
   unsigned long long ngrid_version[2] = {0, 0};
   double ngrid_cut[2] = {0.0, 0.0};

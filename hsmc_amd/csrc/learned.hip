@@ -1051,6 +1051,8 @@ __global__ void __launch_bounds__(256) k_sum_fixed(const double *__restrict__ pa
 
 // Build a CSR grid of one replica (side >= cut) into the given scratch slots.
 int build_ngrid(hsmc_gpu_ctx *ctx, int replica, double cut, int slot0, NGrid &g);
+int build_ngrid_pair(hsmc_gpu_ctx *ctx, int replica, double cut_a, double cut_b, NGrid &ga,
+                     NGrid &gb);
 
 }  // namespace
 
@@ -1103,6 +1105,52 @@ __global__ void k_grid_scatter2(const double *__restrict__ x, const double *__re
   pz[pos] = z[i];
 }
 
+// Two CSR grids of one replica filed in one pass: grid a's cells are the
+// entries [0, Ca) of cnt and grid b's the entries [Ca, Ca + Cb), so one
+// prefix sum over both gives grid a's starts in [0, n] and grid b's in
+// [n, 2n], and both grids index one position array of 2n entries.
+struct CellDims {
+  int nx, ny, nz;
+  double sx, sy, sz;
+};
+
+__device__ __forceinline__ int cell_in(const CellDims &d, double x, double y, double z) {
+  return (plain_cell(x, d.sx, d.nx) * d.ny + plain_cell(y, d.sy, d.ny)) * d.nz +
+         plain_cell(z, d.sz, d.nz);
+}
+
+__global__ void k_grid_count_pair(const double *__restrict__ x, const double *__restrict__ y,
+                                  const double *__restrict__ z, int n, CellDims a, CellDims b,
+                                  int ca, int *__restrict__ cnt, int *__restrict__ cellof) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double xi = x[i], yi = y[i], zi = z[i];
+  const int c0 = cell_in(a, xi, yi, zi), c1 = ca + cell_in(b, xi, yi, zi);
+  cellof[i] = c0;
+  cellof[n + i] = c1;
+  atomicAdd(&cnt[c0], 1);
+  atomicAdd(&cnt[c1], 1);
+}
+
+__global__ void k_grid_scatter_pair(const double *__restrict__ x, const double *__restrict__ y,
+                                    const double *__restrict__ z, int n,
+                                    const int *__restrict__ cellof, const int *__restrict__ start,
+                                    int *__restrict__ fill, double *__restrict__ px,
+                                    double *__restrict__ py, double *__restrict__ pz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double xi = x[i], yi = y[i], zi = z[i];
+  const int c0 = cellof[i], c1 = cellof[n + i];
+  const int s0 = start[c0], s1 = start[c1];
+  const int p0 = s0 + atomicAdd(&fill[c0], 1), p1 = s1 + atomicAdd(&fill[c1], 1);
+  px[p0] = xi;
+  py[p0] = yi;
+  pz[p0] = zi;
+  px[p1] = xi;
+  py[p1] = yi;
+  pz[p1] = zi;
+}
+
 int grid_cells_for2(double L, double cut) {
   int n = (int)floor(L / cut);
   if (n < 1) n = 1;
@@ -1111,15 +1159,71 @@ int grid_cells_for2(double L, double cut) {
 }
 
 // scratch slots 24..43 belong to this file (observables.hip uses 0..13,
-// ctx.hip 21..22, sweep.hip 23): descriptor grid 24..26, overlap grid 27..29,
+// ctx.hip 21..22, sweep.hip 23): descriptor grid 24..26, the insertion call's
+// descriptor and overlap grids built together 27..29,
 // rows / scores 30, targets / block partials 31, cumulative mass 32,
 // sums 34, per-block insertion weights 35, cumulative-mass block sums 36,
 // grid-scan partials 37, voxel scores 38, score block reductions 39,
 // free-voxel tile slots and list 40, tile counts, offsets and scan space 41,
 // blocked-voxel flags 42, softmax statistics 43.
-enum { L_GRID_A = 24, L_GRID_B = 27, L_X = 30, L_T = 31, L_CDF = 32, L_ACC = 34, L_PART = 35,
+enum { L_GRID_A = 24, L_GRID_PAIR = 27, L_X = 30, L_T = 31, L_CDF = 32, L_ACC = 34, L_PART = 35,
        L_BSUM = 36, L_SCANP = 37, L_F = 38, L_FRED = 39, L_LIST = 40, L_NFREE = 41, L_FLAG = 42,
        L_STATS = 43 };
+
+void set_grid(NGrid &g, const RepHost &h, double cut, const int *start, const double *pos,
+              size_t stride) {
+  g.nx = grid_cells_for2(h.p.lx, cut);
+  g.ny = grid_cells_for2(h.p.ly, cut);
+  g.nz = grid_cells_for2(h.p.lz, cut);
+  g.sx = h.p.lx / g.nx; g.sy = h.p.ly / g.ny; g.sz = h.p.lz / g.nz;
+  g.lx = h.p.lx; g.ly = h.p.ly; g.lz = h.p.lz;
+  g.hx = h.p.lx / 2.0; g.hy = h.p.ly / 2.0; g.hz = h.p.lz / 2.0;
+  g.start = start;
+  g.px = pos; g.py = pos + stride; g.pz = pos + 2 * stride;
+}
+
+// Descriptor grid (cut_a) and overlap grid (cut_b) of `replica` in one set
+// of launches: the cache entry 1 of the context's grid cache, keyed by
+// replica and configuration version (the cuts are the insertion call's
+// constants).
+int build_ngrid_pair(hsmc_gpu_ctx *ctx, int replica, double cut_a, double cut_b, NGrid &ga,
+                     NGrid &gb) {
+  const RepHost &h = ctx->reps[replica];
+  const int N = h.p.n_part;
+  const long Ca = (long)grid_cells_for2(h.p.lx, cut_a) * grid_cells_for2(h.p.ly, cut_a) *
+                  grid_cells_for2(h.p.lz, cut_a);
+  const long Cb = (long)grid_cells_for2(h.p.lx, cut_b) * grid_cells_for2(h.p.ly, cut_b) *
+                  grid_cells_for2(h.p.lz, cut_b);
+  const long C = Ca + Cb;
+  if (C + 1 > (1L << 31) - 1 || 2L * N > (1L << 31) - 1) return HSMC_ERR_NOT_SUPPORTED;
+  const bool cached = ctx->ngrid_replica[1] == replica && ctx->ngrid_version[1] == ctx->version &&
+                      ctx->ngrid_cut[1] == cut_a + cut_b;
+  int *cnt = (int *)scratch(ctx, L_GRID_PAIR + 0, sizeof(int) * 2 * (C + 1) + sizeof(int) * 2 * N);
+  double *pos = (double *)scratch(ctx, L_GRID_PAIR + 1, sizeof(double) * 6 * (size_t)N);
+  int *start = (int *)scratch(ctx, L_GRID_PAIR + 2, sizeof(int) * (C + 1));
+  int *part = (int *)scratch(ctx, L_SCANP, sizeof(int) * scan_workspace_ints(C));
+  if (!cnt || !pos || !start || !part) return HSMC_ERR_NO_MEMORY;
+  int *fill = cnt + (C + 1);
+  int *cellof = fill + (C + 1);
+  const size_t stride = 2 * (size_t)N;
+  set_grid(ga, h, cut_a, start, pos, stride);
+  set_grid(gb, h, cut_b, start + Ca, pos, stride);
+  if (cached) return HSMC_OK;
+  const CellDims da{ga.nx, ga.ny, ga.nz, ga.sx, ga.sy, ga.sz};
+  const CellDims db{gb.nx, gb.ny, gb.nz, gb.sx, gb.sy, gb.sz};
+  const double *x = ctx->d_x + h.poff, *y = ctx->d_y + h.poff, *z = ctx->d_z + h.poff;
+  HSMC_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(int) * 2 * (C + 1), ctx->stream));
+  hipLaunchKernelGGL(k_grid_count_pair, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, x, y, z,
+                     N, da, db, (int)Ca, cnt, cellof);
+  exclusive_scan(ctx->stream, cnt, (int)C, start, part);
+  hipLaunchKernelGGL(k_grid_scatter_pair, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, x, y,
+                     z, N, cellof, start, fill, pos, pos + stride, pos + 2 * stride);
+  HSMC_HIP(ctx, hipGetLastError());
+  ctx->ngrid_replica[1] = replica;
+  ctx->ngrid_version[1] = ctx->version;
+  ctx->ngrid_cut[1] = cut_a + cut_b;
+  return HSMC_OK;
+}
 
 int build_ngrid(hsmc_gpu_ctx *ctx, int replica, double cut, int slot0, NGrid &g) {
   const RepHost &h = ctx->reps[replica];
@@ -1127,7 +1231,7 @@ int build_ngrid(hsmc_gpu_ctx *ctx, int replica, double cut, int slot0, NGrid &g)
   const int nx = grid_cells_for2(h.p.lx, cut), ny = grid_cells_for2(h.p.ly, cut),
             nz = grid_cells_for2(h.p.lz, cut);
   const long C = (long)nx * ny * nz;
-  const int which = slot0 == L_GRID_A ? 0 : 1;
+  const int which = 0;  // entry 1 is the paired build's
   const bool cached = ctx->ngrid_replica[which] == replica &&
                       ctx->ngrid_version[which] == ctx->version && ctx->ngrid_cut[which] == cut;
   int *cnt = (int *)scratch(ctx, slot0 + 0, sizeof(int) * 2 * (C + 1) + sizeof(int) * N);
@@ -1487,9 +1591,7 @@ int hsmc_gpu_widom_learned(hsmc_gpu_mlp *m, int replica, double voxel, double mi
   hsmc_gpu_ctx *ctx = m->ctx;
   if (replica < 0 || replica >= ctx->R) return HSMC_ERR_BAD_ARG;
   NGrid g, g1;
-  int st = build_ngrid(ctx, replica, kDescCut, L_GRID_A, g);
-  if (st) return st;
-  st = build_ngrid(ctx, replica, 1.0 + 1e-9, L_GRID_B, g1);
+  int st = build_ngrid_pair(ctx, replica, kDescCut, 1.0 + 1e-9, g, g1);
   if (st) return st;
   const Vox vx = make_vox(ctx->reps[replica], voxel);
   if (vx.V > (1L << 31) - 1) return HSMC_ERR_NOT_SUPPORTED;
