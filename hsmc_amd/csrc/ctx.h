@@ -6,13 +6,16 @@
 //   prep             int32[P]    replica of each particle
 //   cnt              int32[2*C]  occupancy of each sweep cell (two buffers,
 //                                alternating by sweep parity)
-//   slot             SlotRec[cap*C] position copy and particle id in each
+//   slot             SlotRec[2*cap*C] position copy and particle id in each
 //                                cell slot (32 B, one cache-line request each),
 //                                slot-major: slot q of cell c is record q*C + c,
 //                                so the first occupants of z-adjacent cells
-//                                share cache lines
+//                                share cache lines (one replica: two buffers,
+//                                by sweep parity, as for the counts)
 //   reps             RepDev[R]   box, grid and RNG key of each replica
-//   swp              SweepDev[R] the current sweep's grid shift and sublattice order
+//   swp              SweepDev[max(R, 520)] the sweep's grid shift and
+//                                sublattice order per replica (one replica:
+//                                entry k for the k-th sweep of a batch)
 //   aoff             int32[R+1]  prefix of active cells per phase, each
 //                                replica's range padded to a multiple of 8
 //   arep             int32[A]    replica owning each active cell
@@ -46,6 +49,10 @@ struct RepDev {
   int poff, coff, aoff;  // offsets into particle, cell and active-cell arrays
   uint32_t k0, k1;       // Philox key
 };
+
+// Entries of the one-replica sweep table: one per sweep of a batch (at most
+// 512 sweeps per graph) and one past the last.
+constexpr int kSweepTable = 520;
 
 // What one sweep of a replica draws before any move: the grid shift and the
 // order of the 8 sublattices (nibble p = sublattice of phase p).  k_rebin
@@ -83,7 +90,10 @@ struct hsmc_gpu_ctx {
   size_t cell_alloc = 0;  // cells allocated (>= Ctot)
   hsmc::RepDev *d_reps = nullptr;
   hsmc::RepDev rep0{};  // host copy of replica 0's record (a kernel argument when R == 1)
-  hsmc::SweepDev *d_swp = nullptr;  // R entries: the current sweep's shift and order
+  // shift and order: entry r of replica r; one replica: entry k of the k-th
+  // sweep of the last batch (the last sweep's entry is swp_last)
+  hsmc::SweepDev *d_swp = nullptr;
+  int swp_last = 0;
   int *d_aoff = nullptr;
   int *d_arep = nullptr;  // replica of each active cell (Atot entries)
   size_t arep_alloc = 0;

@@ -228,7 +228,7 @@ int layout(hsmc_gpu_ctx *ctx) {
     ctx->d_cacc = ctx->d_cleft = nullptr;
     size_t alloc = need + need / 8 + 64;
     HSMC_HIP(ctx, hipMalloc(&ctx->d_cnt, sizeof(int) * 2 * alloc));
-    HSMC_HIP(ctx, hipMalloc(&ctx->d_slot, sizeof(SlotRec) * alloc * cap));
+    HSMC_HIP(ctx, hipMalloc(&ctx->d_slot, sizeof(SlotRec) * 2 * alloc * cap));  // per sweep parity
     HSMC_HIP(ctx, hipMalloc(&ctx->d_cacc, sizeof(unsigned int) * alloc));
     HSMC_HIP(ctx, hipMalloc(&ctx->d_cleft, sizeof(unsigned int) * alloc));
     ctx->cell_alloc = alloc;
@@ -367,7 +367,7 @@ int hsmc_gpu_create(hsmc_gpu_ctx **out, int device, int n_replicas,
   if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess)
     return fail(HSMC_ERR_HIP);
   if (hipMalloc(&ctx->d_reps, sizeof(RepDev) * n_replicas) != hipSuccess ||
-      hipMalloc(&ctx->d_swp, sizeof(SweepDev) * n_replicas) != hipSuccess ||
+      hipMalloc(&ctx->d_swp, sizeof(SweepDev) * std::max(n_replicas, kSweepTable)) != hipSuccess ||
       hipMalloc(&ctx->d_aoff, sizeof(int) * (n_replicas + 1)) != hipSuccess ||
       hipMalloc(&ctx->d_prep, sizeof(int) * ptot) != hipSuccess ||
       hipMalloc(&ctx->d_x, sizeof(double) * ptot) != hipSuccess ||
@@ -388,7 +388,7 @@ int hsmc_gpu_create(hsmc_gpu_ctx **out, int device, int n_replicas,
   }
   if (hipMemcpy(ctx->d_prep, prep.data(), sizeof(int) * ptot, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(ctx->d_sweep, 0, sizeof(unsigned long long)) != hipSuccess ||
-      hipMemset(ctx->d_swp, 0, sizeof(SweepDev) * n_replicas) != hipSuccess ||
+      hipMemset(ctx->d_swp, 0, sizeof(SweepDev) * std::max(n_replicas, kSweepTable)) != hipSuccess ||
       hipMemset(ctx->d_err, 0, sizeof(unsigned int)) != hipSuccess ||
       hipMemset(ctx->d_bar, 0, 2 * sizeof(unsigned int)) != hipSuccess ||
       hipMemset(ctx->d_x, 0, sizeof(double) * ptot) != hipSuccess ||

@@ -50,9 +50,9 @@ constexpr uint32_t kNptTag = 0x4E505400u;  // "NPT\0"
 // higher ids (the scaled distance is the same from either end, as
 // fl(a - b) = -fl(b - a)).  A thread stops once its replica has an overlap.
 __global__ void __launch_bounds__(256)
-k_vol_overlap(const RepDev *__restrict__ reps, const SweepDev *__restrict__ swp,
+k_vol_overlap(const RepDev *__restrict__ reps, const SweepDev *__restrict__ swp, int R,
               const int *__restrict__ prep, const int *__restrict__ blk, int C,
-              const int *__restrict__ cnt, const SlotRec *__restrict__ slot, int cap,
+              const int *__restrict__ cnt, const SlotRec *__restrict__ slot_all, int cap,
               const unsigned long long *__restrict__ sweep_next, const double *__restrict__ x,
               const double *__restrict__ y, const double *__restrict__ z,
               const double *__restrict__ sfr, int *__restrict__ flags) {
@@ -65,7 +65,11 @@ k_vol_overlap(const RepDev *__restrict__ reps, const SweepDev *__restrict__ swp,
   if (__hip_atomic_load(&flags[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const unsigned long long sweep = *sweep_next - 1ull;
   const int *cntb = cnt + (size_t)(sweep & 1ull) * C;
-  // the last sweep's grid shift, as its rebin drew it
+  // one replica keeps slot records and table entries per sweep parity
+  const int sb = R == 1 ? (int)(sweep & 1ull) : 0;
+  const SlotRec *slot = slot_all + (size_t)sb * cap * C;
+  // the last sweep's grid shift, as its rebin drew it (swp points at the
+  // last sweep's entry when there is one replica)
   const double shx = swp[r].shx, shy = swp[r].shy, shz = swp[r].shz;
   const double ax = x[i], ay = y[i], az = z[i];
   const int cx = shifted_cell(ax, shx, rp.lx, rp.ix, rp.gx);
@@ -206,7 +210,7 @@ int hsmc_gpu_any_overlap_scaled_batch(hsmc_gpu_ctx *ctx, const double *sf, uint8
   HSMC_HIP(ctx, hipMemcpyAsync(d_blk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice, ctx->stream));
   hipLaunchKernelGGL(k_zero_flags, dim3((R + 255) / 256), dim3(256), 0, ctx->stream, d_flag, R);
   hipLaunchKernelGGL(k_vol_overlap, dim3((unsigned)blk.size()), dim3(256), 0, ctx->stream, ctx->d_reps,
-                     ctx->d_swp, ctx->d_prep, d_blk, (int)ctx->Ctot, ctx->d_cnt, ctx->d_slot,
+                     ctx->d_swp + (ctx->R == 1 ? ctx->swp_last : 0), ctx->R, ctx->d_prep, d_blk, (int)ctx->Ctot, ctx->d_cnt, ctx->d_slot,
                      ctx->cap, ctx->d_sweep, ctx->d_x, ctx->d_y, ctx->d_z, d_sf, d_flag);
   HSMC_HIP(ctx, hipGetLastError());
   std::vector<int> hf(R);

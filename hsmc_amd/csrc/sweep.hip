@@ -14,6 +14,12 @@
 //      (moves.c:52-54), wrapped by apply_pbc (moves.c:215-226) and tested
 //      for overlap with every other sphere in the 27 neighbouring cells with
 //      the reference's FP64 arithmetic (moves.c:400-431).
+// With one replica, the sweeps of a batch after the first are chained: the
+// last phase of each sweep files every sphere into the next sweep's shifted
+// grid (at its final position), so those sweeps need no k_rebin, and the
+// batch's first rebin draws the shift and order of all its sweeps.  The cell
+// list a sweep reads is the same either way, up to the order of a cell's
+// slots, which no result depends on (occupants are ranked by id).
 // The random numbers come from Philox4x32-10 keyed by the replica seed with
 // counters (cell, attempt, sweep), so a run is reproducible bit for bit and
 // equals the CPU restatement in the oracle.
@@ -72,14 +78,16 @@ __device__ __forceinline__ uint32_t sweep_order(const RepDev &rp, unsigned long 
 
 // Work item i of a sweep's rebin: clear cell i of the next sweep's count
 // buffer, file particle i into its cell of the shifted grid, and draw
-// replica i's shift and sublattice order.  The particle's position (and, for
+// replica i's shift and sublattice order.  Counts have one buffer per sweep
+// parity; with one replica the slot records do too, so that a sweep's last
+// phase can file the next sweep's list while it reads its own.  The particle's position (and, for
 // several replicas, its replica index) is loaded first, so the shift's draw
 // runs while the loads are in flight; one replica's record is the kernel
 // argument (no per-lane loads of it); replica i's table entry is drawn last,
 // while the slot index's atomic is in flight.
 template <bool ONE>
 __device__ __forceinline__ void rebin_one(int i, const RepDev *__restrict__ reps, const RepDev &rp0,
-                                          SweepDev *__restrict__ swp, int R,
+                                          SweepDev *__restrict__ swp, int R, int kidx, int nsw,
                                           const int *__restrict__ prep, int P, int C,
                                           const double *__restrict__ x, const double *__restrict__ y,
                                           const double *__restrict__ z, int *__restrict__ cnt,
@@ -106,31 +114,38 @@ __device__ __forceinline__ void rebin_one(int i, const RepDev *__restrict__ reps
       rec.z = pz;
       rec.id = i - rp.poff;
       rec.pad = 0;
-      slot[(size_t)sl * C + c] = rec;  // slot-major: slot q of every cell is one array
+      // slot-major: slot q of every cell is one array; with one replica,
+      // one record buffer per sweep parity, as for the counts
+      slot[((size_t)(ONE ? buf : 0) * cap + sl) * C + c] = rec;
     } else {
       atomicOr(err, kErrOverflow);
     }
   }
-  if (i < R) {  // replica i's shift and sublattice order, read by this sweep's phases
+  // the shift and sublattice order read by the phases: entry i of replica i
+  // for several replicas; with one replica, entries kidx .. kidx + nsw - 1
+  // for this sweep and the nsw - 1 after it (a batch of chained sweeps)
+  if (i < (ONE ? nsw : R)) {
     const RepDev ri = ONE ? rp0 : reps[i];
+    const unsigned long long si = ONE ? sweep + (unsigned long long)i : sweep;
     SweepDev d;
-    sweep_shift(ri, sweep, d.shx, d.shy, d.shz);
-    d.order = sweep_order(ri, sweep);
+    sweep_shift(ri, si, d.shx, d.shy, d.shz);
+    d.order = sweep_order(ri, si);
     d.pad = 0u;
-    swp[i] = d;
+    swp[ONE ? kidx + i : i] = d;
   }
 }
 
 template <bool ONE>
 __global__ void __launch_bounds__(256)
 k_rebin(const RepDev *__restrict__ reps, const RepDev rp0, SweepDev *__restrict__ swp, int R,
+        int kidx, int nsw,
         const int *__restrict__ prep, int P, int C,
         const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
         int *__restrict__ cnt, SlotRec *__restrict__ slot, int cap,
         const unsigned long long *__restrict__ sweep_base, int k, unsigned *__restrict__ err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const unsigned long long sweep = *sweep_base + (unsigned long long)(long long)k;
-  rebin_one<ONE>(i, reps, rp0, swp, R, prep, P, C, x, y, z, cnt, slot, cap, sweep, err);
+  rebin_one<ONE>(i, reps, rp0, swp, R, kidx, nsw, prep, P, C, x, y, z, cnt, slot, cap, sweep, err);
 }
 
 // Value of v in lane t of the wave, for a t that is the same in every lane:
@@ -144,26 +159,35 @@ __device__ __forceinline__ double wave_lane(double v, int t) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// A sphere that one lane of a last phase files into the next sweep's list:
+// an occupant of the lane's active cell at its final position.
+struct FileReq {
+  double x, y, z;
+  int id;
+  bool has;
+};
+
 // A lane's copies of its share of the stencil list (entries lane,
 // lane + LPC, ...): a vector value, so variable indexing stays in registers.
 template <int M>
 using dvec = double __attribute__((ext_vector_type(M)));
 
-// Occupancy: at least 6 waves per SIMD (at most 80 VGPRs) for stencil lists
-// of up to 192 entries, 4 for the longest lists and for one replica (whose
-// few active cells never fill the device).
 // One active cell's attempts in one sublattice phase: the work of one group
 // of LPC lanes.  a is the active-cell index, a0 the first active cell of the
 // group's workgroup (of cells_per_block consecutive cells), Lm the group's
-// LDS region.
-template <int CAP, int LPC, int LCAP, bool ONE>
+// LDS region.  One replica: sw0 is the sweep's table entry, and with FILE
+// set (the last phase of a sweep whose successor has no rebin) each lane
+// returns in req the occupant of the cell it files into the next sweep's
+// cell list, at its final position.
+template <int CAP, int LPC, int LCAP, bool ONE, bool FILE>
 __device__ __forceinline__ void phase_cell(
     int a, int a0, int lane, int *Lm, const RepDev *__restrict__ reps, const RepDev &rp0,
-    const SweepDev *__restrict__ swp, const SweepDev &sw0, const int *__restrict__ arep, int A,
-    int phase, unsigned long long sweep, double *__restrict__ x, double *__restrict__ y,
-    double *__restrict__ z, const int *__restrict__ cnt, int C, SlotRec *__restrict__ slot,
+    const SweepDev *__restrict__ swp, const SweepDev &sw0, const int *__restrict__ arep, int R,
+    int A, int phase, unsigned long long sweep,
+    double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
+    int *__restrict__ cnt, int C, SlotRec *__restrict__ slot_all,
     unsigned int *__restrict__ cacc, unsigned int *__restrict__ cleft,
-    unsigned int *__restrict__ err) {
+    unsigned int *__restrict__ err, FileReq &req) {
   // The stencil list (the spheres of the 27 neighbour cells, at most LCAP, a
   // bound the host derives from the cell side) lives in registers: entry e
   // is held by lane e % LPC as its copy e / LPC.  LDS holds the slot of each
@@ -175,7 +199,9 @@ __device__ __forceinline__ void phase_cell(
   double *Op = reinterpret_cast<double *>(Lo + CAP);   // occupants' positions (x, y, z)
   double *Od = Op + 3 * CAP;                           // each attempt's displacement
 
-  const int *cntb = cnt + (size_t)(sweep & 1ull) * C;
+  const int par = (int)(sweep & 1ull);
+  const int *cntb = cnt + (size_t)par * C;
+  SlotRec *slot = slot_all + (size_t)(ONE ? par : 0) * CAP * C;
   // the replica's record (a kernel argument when there is one replica) and
   // the sweep's shift and sublattice order, which the rebin drew (loaded by
   // the caller when there is one replica).  Each replica's active cells are
@@ -413,6 +439,29 @@ __device__ __forceinline__ void phase_cell(
       }
     }
   }
+  if constexpr (ONE && FILE) {
+    // the occupants (list entries oc .. oc + nc - 1) at their final
+    // positions: the nc <= CAP <= LPC entries lie in distinct lanes, so a
+    // lane has at most one, picked by selects
+    int jsel = -1;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      const int q = lane + j * LPC;
+      if (q >= oc && q < oc + nc) jsel = j;
+    }
+    double fx = px[0], fy = py[0], fz = pz[0];
+#pragma unroll
+    for (int j = 1; j < M; ++j) {
+      fx = j == jsel ? (double)px[j] : fx;
+      fy = j == jsel ? (double)py[j] : fy;
+      fz = j == jsel ? (double)pz[j] : fz;
+    }
+    req.x = fx;
+    req.y = fy;
+    req.z = fz;
+    req.has = jsel >= 0;
+    req.id = jsel >= 0 ? Sp[lane + jsel * LPC - oc] : 0;
+  }
   // move counters per active-cell index (the cells of a workgroup share a
   // cache line); the adds never contend and their results are not waited for
   if (lane == 13) {
@@ -421,27 +470,106 @@ __device__ __forceinline__ void phase_cell(
   }
 }
 
+// The filing of a one-replica last phase (MODE 2: the next sweep has no
+// rebin).  Lane g of the launch files particle g, unless it lies in one of
+// this phase's active cells, and the occupant its cell gave it (req).  An
+// active cell's spheres never leave it (a move out is rejected, axis by
+// axis), so particle g's position, read whether or not its cell has moved it
+// yet, gives the same answer to that test.  Both atomics of a lane are
+// issued before either result is used, so a lane waits one round trip.
+template <int CAP>
+__device__ __forceinline__ void file_next_sweep(int g, int P, int C, int phase, const RepDev &rp,
+                                                const SweepDev &cur, const SweepDev &nxt,
+                                                double ax, double ay, double az,
+                                                const FileReq &req, unsigned long long sweep,
+                                                int *__restrict__ cnt,
+                                                SlotRec *__restrict__ slot_all,
+                                                unsigned int *__restrict__ err) {
+  const int par = (int)(sweep & 1ull);
+  int *cntn = cnt + (size_t)(par ^ 1) * C;
+  SlotRec *slotn = slot_all + (size_t)(par ^ 1) * CAP * C;
+  const int colour = (int)((cur.order >> (4 * phase)) & 0xFu);
+  const int cx = shifted_cell(ax, cur.shx, rp.lx, rp.ix, rp.gx);
+  const int cy = shifted_cell(ay, cur.shy, rp.ly, rp.iy, rp.gy);
+  const int cz = shifted_cell(az, cur.shz, rp.lz, rp.iz, rp.gz);
+  const bool active = (((cx ^ colour) | (cy ^ (colour >> 1)) | (cz ^ (colour >> 2))) & 1) == 0;
+  const bool af = g < P && !active;
+  auto next_cell = [&](double px, double py, double pz) {
+    return rp.coff +
+           (shifted_cell(px, nxt.shx, rp.lx, rp.ix, rp.gx) * rp.gy +
+            shifted_cell(py, nxt.shy, rp.ly, rp.iy, rp.gy)) * rp.gz +
+           shifted_cell(pz, nxt.shz, rp.lz, rp.iz, rp.gz);
+  };
+  const int ca = next_cell(ax, ay, az), ct = next_cell(req.x, req.y, req.z);
+  int s1 = 0, s2 = 0;
+  if (af || req.has) {
+    s1 = atomicAdd(&cntn[af ? ca : ct], 1);
+    if (af && req.has) s2 = atomicAdd(&cntn[ct], 1);
+  }
+  const int sa = s1, st = af ? s2 : s1;
+  auto put = [&](int sl, int c, double px, double py, double pz, int id) {
+    if (sl < CAP) {
+      SlotRec rec;
+      rec.x = px;
+      rec.y = py;
+      rec.z = pz;
+      rec.id = id;
+      rec.pad = 0;
+      slotn[(size_t)sl * C + c] = rec;
+    } else {
+      atomicOr(err, kErrOverflow);
+    }
+  };
+  if (af) put(sa, ca, ax, ay, az, g - rp.poff);
+  if (req.has) put(st, ct, req.x, req.y, req.z, req.id);
+}
+
+// One sublattice phase: each workgroup takes cells_per_block active cells.
+// One replica: the sweep's table entry is entry k (its offset in the batch)
+// and the next sweep's entry k + 1.  MODE 1 (the first phase of a sweep with
+// no rebin): lane g also clears the next sweep's count of cell g, the
+// rebin's clear.  MODE 2 (the last phase of a sweep whose successor has no
+// rebin): lane g files particle g and its cell's occupant into the next
+// sweep's list.  The host checks that the lanes cover C and P.  Each mode is
+// its own kernel, so the plain phases carry none of this code.
 // Occupancy: at least 6 waves per SIMD (at most 80 VGPRs) for stencil lists
 // of up to 192 entries, 4 for the longest lists and for one replica (whose
 // few active cells never fill the device).
-template <int CAP, int LPC, int LCAP, bool ONE>
+template <int CAP, int LPC, int LCAP, bool ONE, int MODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!ONE && LCAP <= 192 ? 6 : 4)))
-k_phase(const RepDev *__restrict__ reps, const RepDev rp0, const SweepDev *__restrict__ swp,
-        const int *__restrict__ arep, int R, int A, int phase,
-        int k, const unsigned long long *__restrict__ sweep_base, double *__restrict__ x,
-        double *__restrict__ y, double *__restrict__ z, const int *__restrict__ cnt, int C,
-        SlotRec *__restrict__ slot, unsigned int *__restrict__ cacc,
-        unsigned int *__restrict__ cleft, unsigned int *__restrict__ err) {
+k_phase(const unsigned long long *__restrict__ sweep_base, const SweepDev *__restrict__ swp,
+        const RepDev *__restrict__ reps, const RepDev rp0, const int *__restrict__ arep, int R,
+        int P, int A, int phase, int k,
+        double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
+        int *__restrict__ cnt, int C, SlotRec *__restrict__ slot,
+        unsigned int *__restrict__ cacc, unsigned int *__restrict__ cleft,
+        unsigned int *__restrict__ err) {
   extern __shared__ __attribute__((aligned(16))) int smem_i[];
+  SweepDev sw0{}, swn0{};
+  if constexpr (ONE) sw0 = swp[k];
+  if constexpr (ONE && MODE == 2) swn0 = swp[k + 1];
   const unsigned long long sweep = *sweep_base + (unsigned long long)k;
   const int cpb = blockDim.x / LPC;
-  SweepDev sw0{};
-  if constexpr (ONE) sw0 = swp[0];
+  const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  if constexpr (ONE && MODE == 2) {  // particle g's position, loaded first
+    const int gp = g < P ? g : 0;
+    ax = x[gp];
+    ay = y[gp];
+    az = z[gp];
+  }
   const int lane = threadIdx.x & (LPC - 1);
   const int sub = threadIdx.x / LPC;
-  phase_cell<CAP, LPC, LCAP, ONE>(blockIdx.x * cpb + sub, blockIdx.x * cpb, lane,
-                                  smem_i + (size_t)sub * (LCAP + 14 * CAP), reps, rp0, swp, sw0,
-                                  arep, A, phase, sweep, x, y, z, cnt, C, slot, cacc, cleft, err);
+  FileReq req{};
+  phase_cell<CAP, LPC, LCAP, ONE, MODE == 2>(blockIdx.x * cpb + sub, blockIdx.x * cpb, lane,
+                                             smem_i + (size_t)sub * (LCAP + 14 * CAP), reps, rp0,
+                                             swp, sw0, arep, R, A, phase, sweep, x, y, z, cnt, C,
+                                             slot, cacc, cleft, err, req);
+  if constexpr (ONE && MODE == 1) {
+    if (g < C) cnt[(size_t)((sweep & 1ull) ^ 1ull) * C + g] = 0;
+  }
+  if constexpr (ONE && MODE == 2)
+    file_next_sweep<CAP>(g, P, C, phase, rp0, sw0, swn0, ax, ay, az, req, sweep, cnt, slot, err);
 }
 
 // Barrier of a cooperative launch (every workgroup resident): an arrival
@@ -510,14 +638,16 @@ k_sweeps_persistent(const RepDev *__restrict__ reps, const RepDev rp0, SweepDev 
   for (int s = 0; s < n_sweeps; ++s) {
     const unsigned long long sweep = base + (unsigned long long)s;
     for (int i = gtid; i < work; i += gstride)
-      rebin_one<ONE>(i, reps, rp0, swp, R, prep, P, C, x, y, z, cnt, slot, cap, sweep, err);
+      rebin_one<ONE>(i, reps, rp0, swp, R, s, 1, prep, P, C, x, y, z, cnt, slot, cap, sweep, err);
     if (!grid_barrier(bar, err)) return;
     for (int p = 0; p < 8; ++p) {
       for (int a0 = blockIdx.x * cpb; a0 < A; a0 += gridDim.x * cpb) {
         wave_lds_sync();  // the group's LDS region is reused cell after cell
-        const SweepDev sw0 = ONE ? swp[0] : SweepDev{};
-        phase_cell<CAP, LPC, LCAP, ONE>(a0 + sub, a0, lane, Lm, reps, rp0, swp, sw0, arep, A, p,
-                                        sweep, x, y, z, cnt, C, slot, cacc, cleft, err);
+        const SweepDev sw0 = ONE ? swp[s] : SweepDev{};
+        FileReq req{};
+        phase_cell<CAP, LPC, LCAP, ONE, false>(a0 + sub, a0, lane, Lm, reps, rp0, swp, sw0, arep,
+                                               R, A, p, sweep, x, y, z, cnt, C, slot, cacc, cleft,
+                                               err, req);
       }
       if (!grid_barrier(bar, err)) return;
     }
@@ -539,28 +669,34 @@ k_occupant_picks(uint32_t k0, uint32_t k1, uint32_t cell, unsigned long long swe
   out[t] = occupant_pick(w[0], n, cell, (uint32_t)t, sweep, k0, k1);
 }
 
-using PhaseFn = void (*)(const RepDev *, const RepDev, const SweepDev *, const int *, int, int, int, int,
-                         const unsigned long long *, double *, double *, double *, const int *,
-                         int, SlotRec *, unsigned int *, unsigned int *, unsigned int *);
+using PhaseFn = void (*)(const unsigned long long *, const SweepDev *, const RepDev *, const RepDev,
+                         const int *, int, int, int, int, int, double *,
+                         double *, double *, int *, int, SlotRec *, unsigned int *, unsigned int *,
+                         unsigned int *);
 
-template <int CAP, int LCAP, bool ONE>
+template <int CAP, int LCAP, bool ONE, int MODE>
 PhaseFn phase_fn_l(int lpc) {
-  return lpc == 64 ? k_phase<CAP, 64, LCAP, ONE> : k_phase<CAP, 32, LCAP, ONE>;
+  return lpc == 64 ? k_phase<CAP, 64, LCAP, ONE, MODE> : k_phase<CAP, 32, LCAP, ONE, MODE>;
 }
 
-template <bool ONE>
+template <bool ONE, int MODE>
 PhaseFn phase_fn_o(int cap, int lpc, int lcap) {
   if (cap == 8) {
-    if (lcap <= 128) return phase_fn_l<8, 128, ONE>(lpc);
-    return lcap <= 192 ? phase_fn_l<8, 192, ONE>(lpc) : phase_fn_l<8, 256, ONE>(lpc);
+    if (lcap <= 128) return phase_fn_l<8, 128, ONE, MODE>(lpc);
+    return lcap <= 192 ? phase_fn_l<8, 192, ONE, MODE>(lpc) : phase_fn_l<8, 256, ONE, MODE>(lpc);
   }
-  if (lcap <= 128) return phase_fn_l<16, 128, ONE>(lpc);
-  if (lcap <= 192) return phase_fn_l<16, 192, ONE>(lpc);
-  return phase_fn_l<16, 256, ONE>(lpc);
+  if (lcap <= 128) return phase_fn_l<16, 128, ONE, MODE>(lpc);
+  if (lcap <= 192) return phase_fn_l<16, 192, ONE, MODE>(lpc);
+  return phase_fn_l<16, 256, ONE, MODE>(lpc);
 }
 
-PhaseFn phase_fn(int cap, int lpc, int lcap, bool one) {
-  return one ? phase_fn_o<true>(cap, lpc, lcap) : phase_fn_o<false>(cap, lpc, lcap);
+// mode 1 and 2 (the first and last phases of chained sweeps) exist for one
+// replica only
+PhaseFn phase_fn(int cap, int lpc, int lcap, bool one, int mode = 0) {
+  if (!one) return phase_fn_o<false, 0>(cap, lpc, lcap);
+  return mode == 1   ? phase_fn_o<true, 1>(cap, lpc, lcap)
+         : mode == 2 ? phase_fn_o<true, 2>(cap, lpc, lcap)
+                     : phase_fn_o<true, 0>(cap, lpc, lcap);
 }
 
 using PersistentFn = void (*)(const RepDev *, const RepDev, SweepDev *, const int *, const int *,
@@ -648,27 +784,46 @@ bool tuning_ok(const hsmc_gpu_ctx *ctx, const hsmc_gpu_tuning &t) {
   return true;
 }
 
-// Enqueue the kernels of sweep k of the current batch on stream s.
-void enqueue_sweep(hsmc_gpu_ctx *ctx, int k, hipStream_t s, hipEvent_t *phase_events) {
-  const long work = std::max(ctx->Ptot, ctx->Ctot);
-  hipLaunchKernelGGL(ctx->R == 1 ? k_rebin<true> : k_rebin<false>,
-                     dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, ctx->d_reps,
-                     ctx->rep0, ctx->d_swp, ctx->R, ctx->d_prep, (int)ctx->Ptot, (int)ctx->Ctot, ctx->d_x, ctx->d_y, ctx->d_z,
-                     ctx->d_cnt, ctx->d_slot, ctx->cap,
-                     ctx->d_sweep, k, ctx->d_err);
+// Enqueue the kernels of sweep k of the current batch on stream s.  With
+// `chained` set, the sweep's rebin is left out (the previous sweep's last
+// phase filed its cell list) and its first phase clears the next sweep's
+// counts in the rebin's place; with `file_next` set, its last phase files
+// the next sweep's cell list.  A one-replica rebin draws the table entries
+// of nsw sweeps from this one (k .. k + nsw - 1: the chained sweeps after
+// it read theirs there).
+void enqueue_sweep(hsmc_gpu_ctx *ctx, int k, hipStream_t s, hipEvent_t *phase_events,
+                   bool chained = false, bool file_next = false, int nsw = 1) {
+  const long work = std::max(std::max(ctx->Ptot, ctx->Ctot), (long)nsw);
+  if (!chained)
+    hipLaunchKernelGGL(ctx->R == 1 ? k_rebin<true> : k_rebin<false>,
+                       dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, ctx->d_reps,
+                       ctx->rep0, ctx->d_swp, ctx->R, k, nsw, ctx->d_prep, (int)ctx->Ptot,
+                       (int)ctx->Ctot, ctx->d_x, ctx->d_y, ctx->d_z, ctx->d_cnt, ctx->d_slot,
+                       ctx->cap, ctx->d_sweep, k, ctx->d_err);
   const int lpc = ctx->tune.lanes_per_cell, cpb = ctx->tune.cells_per_block;
+  const int bs = lpc * cpb;
   const unsigned blocks = (unsigned)((ctx->Atot + cpb - 1) / cpb);
   const size_t lds = phase_lds_bytes(ctx->cap, ctx->lcap, cpb);
-  PhaseFn fn = phase_fn(ctx->cap, lpc, ctx->lcap, ctx->R == 1);
   for (int p = 0; p < 8; ++p) {
+    const int mode = p == 0 && chained ? 1 : p == 7 && file_next ? 2 : 0;
+    PhaseFn fn = phase_fn(ctx->cap, lpc, ctx->lcap, ctx->R == 1, mode);
     if (phase_events && p == 0) (void)hipEventRecord(phase_events[0], s);
-    hipLaunchKernelGGL(fn, dim3(blocks), dim3(lpc * cpb), lds, s, ctx->d_reps, ctx->rep0, ctx->d_swp,
-                       ctx->d_arep,
-                       ctx->R, (int)ctx->Atot, p, k, ctx->d_sweep, ctx->d_x, ctx->d_y, ctx->d_z,
-                       ctx->d_cnt, (int)ctx->Ctot, ctx->d_slot, ctx->d_cacc, ctx->d_cleft,
-                       ctx->d_err);
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(bs), lds, s, ctx->d_sweep, ctx->d_swp,
+                       ctx->d_reps, ctx->rep0, ctx->d_arep, ctx->R, (int)ctx->Ptot,
+                       (int)ctx->Atot, p, k, ctx->d_x, ctx->d_y, ctx->d_z,
+                       ctx->d_cnt, (int)ctx->Ctot, ctx->d_slot, ctx->d_cacc,
+                       ctx->d_cleft, ctx->d_err);
     if (phase_events && p == 7) (void)hipEventRecord(phase_events[15], s);
   }
+}
+
+// Whether sweeps can run chained (no rebin; the previous sweep's last phase
+// files the list): one replica, and at least as many phase lanes as cells
+// (whose next counts they clear) and particles (which they file).
+bool can_chain(const hsmc_gpu_ctx *ctx) {
+  const int lpc = ctx->tune.lanes_per_cell, cpb = ctx->tune.cells_per_block;
+  const long lanes = (ctx->Atot + cpb - 1) / cpb * lpc * cpb;
+  return ctx->R == 1 && lanes >= ctx->Ctot && lanes >= ctx->Ptot;
 }
 
 // Capture K sweeps (and the counter advance) into one graph: the main graph
@@ -683,7 +838,13 @@ int capture_graph(hsmc_gpu_ctx *ctx, int K, bool remainder = false) {
   graph = nullptr;
   (remainder ? ctx->graph_r_sweeps : ctx->graph_sweeps) = 0;
   HSMC_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < K; ++k) enqueue_sweep(ctx, k, ctx->stream, nullptr);
+  // one replica: the batch's first sweep rebins and each later one takes
+  // the cell list its predecessor's last phase filed (with several replicas
+  // the filing's registers would lower the phase kernel's occupancy)
+  const bool chain = can_chain(ctx);
+  for (int k = 0; k < K; ++k)
+    enqueue_sweep(ctx, k, ctx->stream, nullptr, chain && k > 0, chain && k + 1 < K,
+                  chain && k == 0 ? K : 1);
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, ctx->stream, ctx->d_sweep, K);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(ctx->stream, &g);
@@ -727,6 +888,7 @@ int run_sweeps(hsmc_gpu_ctx *ctx, int n) {
       if (*ctx->h_err) return check_device_errors(ctx);
       int st = launch_persistent(ctx, std::min(per, n - done));
       if (st) return st;
+      ctx->swp_last = std::min(per, n - done) - 1;
       HSMC_HIP(ctx, hipMemcpyAsync((void *)ctx->h_err, ctx->d_err, sizeof(unsigned int),
                                    hipMemcpyDeviceToHost, ctx->stream));
     }
@@ -745,6 +907,7 @@ int run_sweeps(hsmc_gpu_ctx *ctx, int n) {
       HSMC_HIP(ctx, hipGraphLaunch(ctx->gexec, ctx->stream));
       HSMC_HIP(ctx, hipMemcpyAsync((void *)ctx->h_err, ctx->d_err, sizeof(unsigned int),
                                    hipMemcpyDeviceToHost, ctx->stream));
+      ctx->swp_last = K - 1;
     }
   }
   const int rem = n - done;
@@ -754,12 +917,16 @@ int run_sweeps(hsmc_gpu_ctx *ctx, int n) {
         same_launch_shape(ctx->graph_r_tune, ctx->tune)) {
       HSMC_HIP(ctx, hipGraphLaunch(ctx->gexec_r, ctx->stream));  // captured by a timing call
     } else {
-      for (int k = 0; k < rem; ++k) enqueue_sweep(ctx, k, ctx->stream, nullptr);
+      const bool chain = can_chain(ctx) && rem < kSweepTable;
+      for (int k = 0; k < rem; ++k)
+        enqueue_sweep(ctx, k, ctx->stream, nullptr, chain && k > 0, chain && k + 1 < rem,
+                      chain && k == 0 ? rem : 1);
       hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, ctx->stream, ctx->d_sweep, rem);
       HSMC_HIP(ctx, hipGetLastError());
     }
     HSMC_HIP(ctx, hipMemcpyAsync((void *)ctx->h_err, ctx->d_err, sizeof(unsigned int),
                                  hipMemcpyDeviceToHost, ctx->stream));
+    ctx->swp_last = rem - 1;
   }
   ctx->cells_version = ctx->version;  // the last sweep's cells hold every sphere
   return HSMC_OK;
@@ -777,10 +944,11 @@ int rebin_last_sweep(hsmc_gpu_ctx *ctx) {
   const long work = std::max(ctx->Ptot, ctx->Ctot);
   hipLaunchKernelGGL(ctx->R == 1 ? k_rebin<true> : k_rebin<false>,
                      dim3((unsigned)((work + 255) / 256)), dim3(256), 0, ctx->stream,
-                     ctx->d_reps, ctx->rep0, ctx->d_swp, ctx->R, ctx->d_prep, (int)ctx->Ptot, (int)ctx->Ctot, ctx->d_x, ctx->d_y,
-                     ctx->d_z, ctx->d_cnt, ctx->d_slot, ctx->cap,
-                     ctx->d_sweep, -1, ctx->d_err);
+                     ctx->d_reps, ctx->rep0, ctx->d_swp, ctx->R, 0, 1, ctx->d_prep, (int)ctx->Ptot,
+                     (int)ctx->Ctot, ctx->d_x, ctx->d_y, ctx->d_z, ctx->d_cnt, ctx->d_slot,
+                     ctx->cap, ctx->d_sweep, -1, ctx->d_err);
   HSMC_HIP(ctx, hipGetLastError());
+  ctx->swp_last = 0;
   ctx->cells_version = ctx->version;
   return HSMC_OK;
 }
@@ -848,6 +1016,7 @@ int hsmc_gpu_time_phase_kernel(hsmc_gpu_ctx *ctx, int n_sweeps, float *avg_ms, i
   }
   for (int i = 0; i < 16; ++i) (void)hipEventDestroy(ev[i]);
   ctx->version++;
+  ctx->swp_last = 0;
   ctx->cells_version = ctx->version;
   *avg_ms = (float)(total / count);
   if (launches) *launches = count;
