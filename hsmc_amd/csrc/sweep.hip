@@ -1080,6 +1080,9 @@ int hsmc_gpu_set_sweep_counter(hsmc_gpu_ctx *ctx, uint64_t sweep) {
   HSMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   // attempts are N * (sweep - sweep_at_reset): move the origin with the counter
   ctx->sweep_at_reset += want - now;
+  // the last sweep's cell list is found by the counter's parity: refile it
+  // before anything reads it
+  ctx->cells_version = 0;
   return HSMC_OK;
 }
 

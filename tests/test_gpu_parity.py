@@ -167,6 +167,29 @@ def test_sweeps_graph_and_direct_agree():
     assert np.array_equal(outs[0], outs[1])
 
 
+@pytest.mark.parametrize("lpc,cpb", [(32, 2), (64, 1)])
+def test_chained_sweeps_equal_rebinned_sweeps(lpc, cpb):
+    """One replica: inside a batch each sweep's last phase files the next
+    sweep's cell list and only the first sweep rebins.  A direct call of
+    530 sweeps (more than the sweep table holds, so every sweep rebins) must
+    equal the same sweeps run as 64-sweep chained graphs plus a chained
+    remainder, bit for bit, counters included."""
+    o = O.System(2, 8, rho=0.85, neigh_dr=1.0, max_part=10, seed=2)
+    o.sweep_nvt(2)
+    conf = o.config()
+    L = o.box()["lx"]
+    outs = []
+    for spg in (0, 64):
+        ctx = gpu.Context([dict(n=len(conf), box=(L, L, L), dr_max=0.1, seed=7)])
+        ctx.set_tuning(lpc, cpb, spg)
+        ctx.upload(conf)
+        ctx.sweeps(530)
+        outs.append((ctx.download(), ctx.counters()))
+        ctx.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1]
+
+
 def test_graph_follows_replica_changes():
     """A captured sweep graph of a one-replica context carries the replica's
     record by value: after a step-size change and after a box rescale the
