@@ -350,6 +350,18 @@ __device__ __forceinline__ void phase_cell(
     }
   }
 
+  // Several replicas: whether every pair this wave tests is free of the box
+  // edge.  A cell index k <= g - 2 of the shifted grid holds positions
+  // x = t + shift with no wrap (t < (g - 1) * side <= L - shift), so an
+  // active cell with every index in [1, g - 3] and its 26 neighbours hold
+  // positions within 3 sides of each other, less than half the box when
+  // g >= 8: there compute_dist's minimum image (moves.c:414-427) changes no
+  // difference.  Trial positions are tested only when they stay in the
+  // active cell.  The multi-replica launches keep 6-7 waves per SIMD and
+  // gain from the shorter test; one replica's 2 waves per SIMD do not.
+  const bool inner_cell = rp.gx >= 8 && rp.gy >= 8 && rp.gz >= 8 && cx >= 1 && cx <= rp.gx - 3 &&
+                          cy >= 1 && cy <= rp.gy - 3 && cz >= 1 && cz <= rp.gz - 3;
+  const bool inner = !ONE && __ballot(!inner_cell) == 0ull;  // the lanes still here
   unsigned int n_acc = 0, n_left = 0;
   unsigned int moved = 0u;  // several replicas: bit j, this lane's copy j moved
   for (int t = 0; t < nc; ++t) {
@@ -382,13 +394,27 @@ __device__ __forceinline__ void phase_cell(
     }
     const int owner = (int)(e % LPC), je = (int)(e / LPC);
     bool ov = false;
+    if (inner) {
+      // no pair of this wave crosses the box edge: the minimum image leaves
+      // every difference as it is, so it is left out (same FP64 operations)
 #pragma unroll
-    for (int j = 0; j < M; ++j) {
-      const int q = lane + j * LPC;
-      if (q < T) {
-        const bool hit =
-            dist2(nx, ny, nz, px[j], py[j], pz[j], rp.lx, rp.ly, rp.lz, rp.hx, rp.hy, rp.hz) < 1.0;
-        ov |= hit & (q != (int)e);
+      for (int j = 0; j < M; ++j) {
+        const int q = lane + j * LPC;
+        if (q < T) {
+          const double dx = nx - px[j], dy = ny - py[j], dz = nz - pz[j];
+          const bool hit = dx * dx + dy * dy + dz * dz < 1.0;
+          ov |= hit & (q != (int)e);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        const int q = lane + j * LPC;
+        if (q < T) {
+          const bool hit =
+              dist2(nx, ny, nz, px[j], py[j], pz[j], rp.lx, rp.ly, rp.lz, rp.hx, rp.hy, rp.hz) < 1.0;
+          ov |= hit & (q != (int)e);
+        }
       }
     }
     const unsigned long long bal = __ballot(ov);
