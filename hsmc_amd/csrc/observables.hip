@@ -323,6 +323,13 @@ k_widom_cells(Grid g, uint32_t k0, uint32_t k1, unsigned long long sample, int p
     const int iz = (int)(c % g.nz);
     const long t2 = c / g.nz;
     const int iy = (int)(t2 % g.ny), ix = (int)(t2 / g.ny);
+    // A cell with every index in [1, g - 2] and its neighbours hold positions
+    // and points within 2 sides of each other, under half the box when
+    // g >= 8: compute_dist's minimum image (moves.c:414-427) changes no
+    // difference there and is left out (the same FP64 operations remain).
+    // The cell is the wave's, so the choice is uniform.
+    const bool inner = g.nx >= 8 && g.ny >= 8 && g.nz >= 8 && ix >= 1 && ix <= g.nx - 2 &&
+                       iy >= 1 && iy <= g.ny - 2 && iz >= 1 && iz <= g.nz - 2;
     // neighbour ranges (distinct cells only on axes of fewer than 3 cells)
     int b = 0, n = 0;
     if (lane < 27) {
@@ -340,8 +347,12 @@ k_widom_cells(Grid g, uint32_t k0, uint32_t k1, unsigned long long sample, int p
       const int v = __shfl_up(incl, d, 64);
       if (lane >= d) incl += v;
     }
-    const int T = __shfl(incl, 26, 64);
-    if (T > kWidomCellsLcap) {  // above the packing bound the host checks: cannot happen
+    // the list length, read into a scalar register so that the point loop
+    // below is a uniform loop; padded to a multiple of 4 with positions no
+    // point can overlap (the loop tests 4 entries between its exit checks)
+    const int T = __builtin_amdgcn_readfirstlane(__shfl(incl, 26, 64));
+    const int T4 = (T + 3) & ~3;
+    if (T4 > kWidomCellsLcap) {  // above the packing bound the host checks: cannot happen
       if (lane == 0) atomicOr(err, kErrListFull);
       continue;
     }
@@ -349,6 +360,11 @@ k_widom_cells(Grid g, uint32_t k0, uint32_t k1, unsigned long long sample, int p
       Lx[o + q] = g.px[b + q];
       Ly[o + q] = g.py[b + q];
       Lz[o + q] = g.pz[b + q];
+    }
+    if (lane < T4 - T) {
+      Lx[T + lane] = 1e300;
+      Ly[T + lane] = 1e300;
+      Lz[T + lane] = 1e300;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -362,11 +378,28 @@ k_widom_cells(Grid g, uint32_t k0, uint32_t k1, unsigned long long sample, int p
       const double qx = ((double)ix + ((double)u[0] + 0.5) * 0x1p-32) * g.sx;
       const double qy = ((double)iy + ((double)u[1] + 0.5) * 0x1p-32) * g.sy;
       const double qz = ((double)iz + ((double)u[2] + 0.5) * 0x1p-32) * g.sz;
-      bool ov = false;
-      for (int e = 0; e < T; ++e) {
-        ov |= dist2(qx, qy, qz, Lx[e], Ly[e], Lz[e], g.lx, g.ly, g.lz, g.hx, g.hy, g.hz) < 1.0;
-        if ((e & 3) == 3 && __all(ov || !mine)) break;
+      // the smallest squared distance so far: a point overlaps a sphere iff
+      // it is below 1 (one v_min per pair in place of a compare and a merge)
+      double m = 1e300;
+      if (inner) {
+        for (int e = 0; e < T4; e += 4) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const double dx = qx - Lx[e + i], dy = qy - Ly[e + i], dz = qz - Lz[e + i];
+            m = fmin(m, dx * dx + dy * dy + dz * dz);
+          }
+          if (__all(m < 1.0 || !mine)) break;
+        }
+      } else {
+        for (int e = 0; e < T4; e += 4) {
+#pragma unroll 1
+          for (int i = 0; i < 4; ++i)
+            m = fmin(m, dist2(qx, qy, qz, Lx[e + i], Ly[e + i], Lz[e + i], g.lx, g.ly, g.lz, g.hx,
+                              g.hy, g.hz));
+          if (__all(m < 1.0 || !mine)) break;
+        }
       }
+      const bool ov = m < 1.0;
       if (mine && !ov) ++h;
     }
     __builtin_amdgcn_wave_barrier();  // the list is rewritten for the next cell
