@@ -551,8 +551,9 @@ __device__ __forceinline__ void file_next_sweep(int g, int P, int C, int phase, 
 }
 
 // One sublattice phase: each workgroup takes cells_per_block active cells.
-// One replica: the sweep's table entry is entry k (its offset in the batch)
-// and the next sweep's entry k + 1.  MODE 1 (the first phase of a sweep with
+// One replica: the sweep's table entry is entry kt (its offset in a chained
+// batch, 0 for a sweep with its own rebin) and the next sweep's entry
+// kt + 1.  MODE 1 (the first phase of a sweep with
 // no rebin): lane g also clears the next sweep's count of cell g, the
 // rebin's clear.  MODE 2 (the last phase of a sweep whose successor has no
 // rebin): lane g files particle g and its cell's occupant into the next
@@ -565,15 +566,15 @@ template <int CAP, int LPC, int LCAP, bool ONE, int MODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!ONE && LCAP <= 192 ? 6 : 4)))
 k_phase(const unsigned long long *__restrict__ sweep_base, const SweepDev *__restrict__ swp,
         const RepDev *__restrict__ reps, const RepDev rp0, const int *__restrict__ arep, int R,
-        int P, int A, int phase, int k,
+        int P, int A, int phase, int k, int kt,
         double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
         int *__restrict__ cnt, int C, SlotRec *__restrict__ slot,
         unsigned int *__restrict__ cacc, unsigned int *__restrict__ cleft,
         unsigned int *__restrict__ err) {
   extern __shared__ __attribute__((aligned(16))) int smem_i[];
   SweepDev sw0{}, swn0{};
-  if constexpr (ONE) sw0 = swp[k];
-  if constexpr (ONE && MODE == 2) swn0 = swp[k + 1];
+  if constexpr (ONE) sw0 = swp[kt];
+  if constexpr (ONE && MODE == 2) swn0 = swp[kt + 1];
   const unsigned long long sweep = *sweep_base + (unsigned long long)k;
   const int cpb = blockDim.x / LPC;
   const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -696,7 +697,7 @@ k_occupant_picks(uint32_t k0, uint32_t k1, uint32_t cell, unsigned long long swe
 }
 
 using PhaseFn = void (*)(const unsigned long long *, const SweepDev *, const RepDev *, const RepDev,
-                         const int *, int, int, int, int, int, double *,
+                         const int *, int, int, int, int, int, int, double *,
                          double *, double *, int *, int, SlotRec *, unsigned int *, unsigned int *,
                          unsigned int *);
 
@@ -816,14 +817,15 @@ bool tuning_ok(const hsmc_gpu_ctx *ctx, const hsmc_gpu_tuning &t) {
 // counts in the rebin's place; with `file_next` set, its last phase files
 // the next sweep's cell list.  A one-replica rebin draws the table entries
 // of nsw sweeps from this one (k .. k + nsw - 1: the chained sweeps after
-// it read theirs there).
+// it read theirs there); a sweep outside a chained run uses entry 0.
 void enqueue_sweep(hsmc_gpu_ctx *ctx, int k, hipStream_t s, hipEvent_t *phase_events,
                    bool chained = false, bool file_next = false, int nsw = 1) {
   const long work = std::max(std::max(ctx->Ptot, ctx->Ctot), (long)nsw);
+  const int kt = chained || file_next || nsw > 1 ? k : 0;  // below kSweepTable - 1
   if (!chained)
     hipLaunchKernelGGL(ctx->R == 1 ? k_rebin<true> : k_rebin<false>,
                        dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, ctx->d_reps,
-                       ctx->rep0, ctx->d_swp, ctx->R, k, nsw, ctx->d_prep, (int)ctx->Ptot,
+                       ctx->rep0, ctx->d_swp, ctx->R, kt, nsw, ctx->d_prep, (int)ctx->Ptot,
                        (int)ctx->Ctot, ctx->d_x, ctx->d_y, ctx->d_z, ctx->d_cnt, ctx->d_slot,
                        ctx->cap, ctx->d_sweep, k, ctx->d_err);
   const int lpc = ctx->tune.lanes_per_cell, cpb = ctx->tune.cells_per_block;
@@ -836,7 +838,7 @@ void enqueue_sweep(hsmc_gpu_ctx *ctx, int k, hipStream_t s, hipEvent_t *phase_ev
     if (phase_events && p == 0) (void)hipEventRecord(phase_events[0], s);
     hipLaunchKernelGGL(fn, dim3(blocks), dim3(bs), lds, s, ctx->d_sweep, ctx->d_swp,
                        ctx->d_reps, ctx->rep0, ctx->d_arep, ctx->R, (int)ctx->Ptot,
-                       (int)ctx->Atot, p, k, ctx->d_x, ctx->d_y, ctx->d_z,
+                       (int)ctx->Atot, p, k, kt, ctx->d_x, ctx->d_y, ctx->d_z,
                        ctx->d_cnt, (int)ctx->Ctot, ctx->d_slot, ctx->d_cacc,
                        ctx->d_cleft, ctx->d_err);
     if (phase_events && p == 7) (void)hipEventRecord(phase_events[15], s);
@@ -933,7 +935,7 @@ int run_sweeps(hsmc_gpu_ctx *ctx, int n) {
       HSMC_HIP(ctx, hipGraphLaunch(ctx->gexec, ctx->stream));
       HSMC_HIP(ctx, hipMemcpyAsync((void *)ctx->h_err, ctx->d_err, sizeof(unsigned int),
                                    hipMemcpyDeviceToHost, ctx->stream));
-      ctx->swp_last = K - 1;
+      ctx->swp_last = can_chain(ctx) ? K - 1 : 0;  // the last sweep's table entry
     }
   }
   const int rem = n - done;
@@ -952,7 +954,7 @@ int run_sweeps(hsmc_gpu_ctx *ctx, int n) {
     }
     HSMC_HIP(ctx, hipMemcpyAsync((void *)ctx->h_err, ctx->d_err, sizeof(unsigned int),
                                  hipMemcpyDeviceToHost, ctx->stream));
-    ctx->swp_last = rem - 1;
+    ctx->swp_last = can_chain(ctx) && rem < kSweepTable ? rem - 1 : 0;
   }
   ctx->cells_version = ctx->version;  // the last sweep's cells hold every sphere
   return HSMC_OK;
