@@ -80,11 +80,11 @@ __device__ __forceinline__ uint32_t sweep_order(const RepDev &rp, unsigned long 
 // buffer, file particle i into its cell of the shifted grid, and draw
 // replica i's shift and sublattice order.  Counts have one buffer per sweep
 // parity; with one replica the slot records do too, so that a sweep's last
-// phase can file the next sweep's list while it reads its own.  The particle's position (and, for
-// several replicas, its replica index) is loaded first, so the shift's draw
-// runs while the loads are in flight; one replica's record is the kernel
-// argument (no per-lane loads of it); replica i's table entry is drawn last,
-// while the slot index's atomic is in flight.
+// phase can file the next sweep's list while it reads its own.  The
+// particle's position (and, for several replicas, its replica index) is
+// loaded first, so the shift's draw runs while the loads are in flight; one
+// replica's record is the kernel argument (no per-lane loads of it); the
+// table entries are drawn last, while the slot index's atomic is in flight.
 template <bool ONE>
 __device__ __forceinline__ void rebin_one(int i, const RepDev *__restrict__ reps, const RepDev &rp0,
                                           SweepDev *__restrict__ swp, int R, int kidx, int nsw,
