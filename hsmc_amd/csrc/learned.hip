@@ -758,49 +758,6 @@ k_voxel_scores_list(NGrid g, Vox vx, float far2, const MlpDev *__restrict__ net,
   score_block(&s, X, H1, ys, red, d, ok, v, f, bmax, bsum);
 }
 
-// Max and sum of exp(f - max) over the scores of list[0..n) per block of
-// 1024 entries, each reduced in a fixed order (strided partials, then a
-// tree), so the totals depend only on the scores and the list's order
-// (fixed by the voxel grid), not on the order in which blocks finished.
-constexpr int kPartVoxels = 1024;
-
-__global__ void __launch_bounds__(256)
-k_score_parts(const float *__restrict__ f, const int *__restrict__ list, long V,
-              float *__restrict__ bmax, float *__restrict__ bsum) {
-  __shared__ float s[256];
-  const long base = (long)blockIdx.x * kPartVoxels;
-  float m = -INFINITY;
-  for (int k = threadIdx.x; k < kPartVoxels; k += 256) {
-    const long v = base + k;
-    if (v < V) m = fmaxf(m, f[list[v]]);
-  }
-  s[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + w]);
-    __syncthreads();
-  }
-  const float bm = s[0];
-  __syncthreads();
-  float e = 0.f;
-  if (bm > -INFINITY) {
-    for (int k = threadIdx.x; k < kPartVoxels; k += 256) {
-      const long v = base + k;
-      if (v < V) e += expf(f[list[v]] - bm);
-    }
-  }
-  s[threadIdx.x] = e;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    bmax[blockIdx.x] = bm;
-    bsum[blockIdx.x] = s[0];
-  }
-}
-
 // Network part of the proposal mass of a free voxel v:
 // mix * softmax(f)_v = mix * exp(f_v - fmax) / sumexp, with st[0] = fmax and
 // st[1] = sumexp (on the device, written by k_reduce_parts).
@@ -1395,16 +1352,14 @@ int launch_proposal_scores(hsmc_gpu_ctx *ctx, int replica, const NGrid &g, const
     return HSMC_OK;
   }
   const long nb = ((long)nf + 255) / 256;
-  const long np = ((long)nf + kPartVoxels - 1) / kPartVoxels;
-  float *b = (float *)scratch(ctx, L_FRED, sizeof(float) * 2 * (size_t)std::max(nb, np));
+  float *b = (float *)scratch(ctx, L_FRED, sizeof(float) * 2 * (size_t)nb);
   if (!b) return HSMC_ERR_NO_MEMORY;
-  // the scoring kernel's per-block reductions are not used: the totals are
-  // taken per 1024 list entries in list order
+  // the softmax totals from the scoring kernel's per-block max and sum of
+  // exp(f - max) (each block's 256 list entries reduced in a fixed order),
+  // combined in a fixed order: they depend only on the scores and the list
   hipLaunchKernelGGL(k_voxel_scores_list, dim3((unsigned)nb), dim3(256), 0, ctx->stream, g, vx,
                      far2_of(g), net, list, (int)nf, f, b, b + nb);
-  hipLaunchKernelGGL(k_score_parts, dim3((unsigned)np), dim3(256), 0, ctx->stream, f, list,
-                     (long)nf, b, b + np);
-  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, ctx->stream, b, b + np, np, stats);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, ctx->stream, b, b + nb, nb, stats);
   HSMC_HIP(ctx, hipGetLastError());
   return HSMC_OK;
 }
