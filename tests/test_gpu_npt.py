@@ -64,6 +64,33 @@ def test_batched_scan_matches_single_replica_scan():
     ctx.close()
 
 
+@pytest.mark.parametrize("spg,sweeps", [(64, 200), (0, 530), (0, 37)])
+def test_one_replica_scan_after_chained_and_rebinned_sweeps(spg, sweeps):
+    """One replica: the batched scan reads the last sweep's cell list (slot
+    records by sweep parity) and its grid shift (the table entry of the last
+    sweep of a chained batch, or entry 0 after rebinned sweeps).  After
+    chained graphs (64-sweep batches and a chained remainder), a long direct
+    call (every sweep rebinned) and a short chained direct call, and after
+    the sweep counter is moved, its decisions equal the single-replica scan's
+    on its own grid."""
+    ctx = make([(6, 0.85, 9, 0.1)])  # 864 spheres, sweep-grid side 1.0055: sf >= 0.9946 on it
+    ctx.set_tuning(32, 2, spg)
+    ctx.sweeps(sweeps)
+    side = min(b / g for b, g in zip(ctx.box(0), ctx.grid(0)))
+    outcomes = set()
+    factors = (1.0, 0.9999, 0.9995, 0.999, 0.998, 0.99, 0.95)
+    for sf in factors:
+        got = ctx.any_overlap_scaled_batch(sf)[0]
+        assert got == ctx.any_overlap_scaled([sf], 0)[0], sf
+        if side >= (1.0 / sf) * (1.0 + 1e-9):  # decided on the sweep grid
+            outcomes.add(int(got))
+    assert outcomes == {0, 1}
+    ctx.set_sweep_counter(12345)  # the last list is found by the counter's parity
+    for sf in factors:
+        assert ctx.any_overlap_scaled_batch(sf)[0] == ctx.any_overlap_scaled([sf], 0)[0], sf
+    ctx.close()
+
+
 NPT_SPECS = [(4, 0.60, 11, 0.12), (5, 0.50, 12, 0.15), (3, 0.70, 13, 0.10)]
 NPT_PRESS = [3.0, 2.0, 4.0]
 NPT_DV = [0.01, 0.02, 0.008]
